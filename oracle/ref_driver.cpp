@@ -1,0 +1,368 @@
+// ref_driver.cpp — test-only C shim over the reference CPU RocketSim.
+//
+// ORACLE / TEST INFRASTRUCTURE.  This file is compiled together with the
+// unmodified reference RocketSim + Bullet 3.24 sources (read in place from the
+// reference tree by oracle/Makefile) into oracle/_ref/librsx_ref.so.  Only the
+// tests, the smoke check and bench.py's cpu_baseline leg load it; the engine
+// never links or calls it.
+//
+// It exposes the reference's Arena API (RocketSim/src/Sim/Arena/Arena.h:25-140)
+// as plain C functions over the shared rsx_* records of include/rsx_types.h:
+//   Arena::Create / AddCar            Arena.cpp:49-61, 568-570
+//   Car::GetState / SetState          Car.cpp:10-36
+//   Ball::GetState / SetState         Ball.cpp:27-49
+//   BoostPad::GetState / SetState     BoostPad.h:44-45
+//   Arena::Step                       Arena.cpp:716-812
+//   Arena::IsBallScored               Arena.cpp:949-957
+//   Arena::ResetToRandomKickoff       Arena.cpp:112-216
+// Meshes are loaded through RocketSim::InitFromMem (RocketSim.cpp:102-212) in
+// numeric file order mesh_0 … mesh_15, which fixes the static-body order that a
+// directory scan would leave to the filesystem.
+
+#include "RocketSim.h"
+#include "../include/rsx_types.h"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <vector>
+
+using namespace RocketSim;
+
+namespace {
+
+struct RefArena {
+    Arena* arena = nullptr;
+    std::vector<Car*> cars;  // creation order: blue cars, then orange cars
+};
+
+Vec to_vec(const rsx_vec3& v) { return Vec(v.x, v.y, v.z); }
+rsx_vec3 from_vec(const Vec& v) { return rsx_vec3{v.x, v.y, v.z}; }
+
+CarControls to_controls(const rsx_controls& c) {
+    CarControls r;
+    r.throttle = c.throttle; r.steer = c.steer;
+    r.pitch = c.pitch; r.yaw = c.yaw; r.roll = c.roll;
+    r.jump = c.jump != 0; r.boost = c.boost != 0; r.handbrake = c.handbrake != 0;
+    return r;
+}
+
+rsx_controls from_controls(const CarControls& c) {
+    rsx_controls r{};
+    r.throttle = c.throttle; r.steer = c.steer;
+    r.pitch = c.pitch; r.yaw = c.yaw; r.roll = c.roll;
+    r.jump = c.jump; r.boost = c.boost; r.handbrake = c.handbrake;
+    return r;
+}
+
+void to_public(const CarState& s, const Car* car, rsx_car_state* o) {
+    memset(o, 0, sizeof(*o));
+    o->pos = from_vec(s.pos);
+    o->rot.forward = from_vec(s.rotMat.forward);
+    o->rot.right = from_vec(s.rotMat.right);
+    o->rot.up = from_vec(s.rotMat.up);
+    o->vel = from_vec(s.vel);
+    o->ang_vel = from_vec(s.angVel);
+    o->boost = s.boost;
+    o->time_spent_boosting = s.timeSpentBoosting;
+    o->jump_time = s.jumpTime;
+    o->flip_time = s.flipTime;
+    o->air_time = s.airTime;
+    o->air_time_since_jump = s.airTimeSinceJump;
+    o->supersonic_time = s.supersonicTime;
+    o->handbrake_val = s.handbrakeVal;
+    o->auto_flip_timer = s.autoFlipTimer;
+    o->auto_flip_torque_scale = s.autoFlipTorqueScale;
+    o->demo_respawn_timer = s.demoRespawnTimer;
+    o->car_contact_cooldown = s.carContact.cooldownTimer;
+    o->flip_rel_torque = from_vec(s.flipRelTorque);
+    o->world_contact_normal = from_vec(s.worldContact.contactNormal);
+    o->is_on_ground = s.isOnGround;
+    o->has_jumped = s.hasJumped;
+    o->has_double_jumped = s.hasDoubleJumped;
+    o->has_flipped = s.hasFlipped;
+    o->is_flipping = s.isFlipping;
+    o->is_jumping = s.isJumping;
+    o->is_supersonic = s.isSupersonic;
+    o->is_auto_flipping = s.isAutoFlipping;
+    o->is_demoed = s.isDemoed;
+    o->world_contact = s.worldContact.hasContact;
+    o->ball_hit_valid = s.ballHitInfo.isValid;
+    o->team = (uint8_t)car->team;
+    for (int i = 0; i < 4; i++) o->wheels_with_contact[i] = s.wheelsWithContact[i];
+    o->id = car->id;
+    o->car_contact_other_id = s.carContact.otherCarID;
+    o->ball_hit_tick = s.ballHitInfo.tickCountWhenHit;
+    o->ball_hit_extra_impulse_tick = s.ballHitInfo.tickCountWhenExtraImpulseApplied;
+    o->ball_hit_rel_pos = from_vec(s.ballHitInfo.relativePosOnBall);
+    o->ball_hit_ball_pos = from_vec(s.ballHitInfo.ballPos);
+    o->ball_hit_extra_vel = from_vec(s.ballHitInfo.extraHitVel);
+    o->last_controls = from_controls(s.lastControls);
+}
+
+CarState from_public(const rsx_car_state* o) {
+    CarState s;
+    s.pos = to_vec(o->pos);
+    s.rotMat = RotMat(to_vec(o->rot.forward), to_vec(o->rot.right), to_vec(o->rot.up));
+    s.vel = to_vec(o->vel);
+    s.angVel = to_vec(o->ang_vel);
+    s.boost = o->boost;
+    s.timeSpentBoosting = o->time_spent_boosting;
+    s.jumpTime = o->jump_time;
+    s.flipTime = o->flip_time;
+    s.airTime = o->air_time;
+    s.airTimeSinceJump = o->air_time_since_jump;
+    s.supersonicTime = o->supersonic_time;
+    s.handbrakeVal = o->handbrake_val;
+    s.autoFlipTimer = o->auto_flip_timer;
+    s.autoFlipTorqueScale = o->auto_flip_torque_scale;
+    s.demoRespawnTimer = o->demo_respawn_timer;
+    s.carContact.cooldownTimer = o->car_contact_cooldown;
+    s.flipRelTorque = to_vec(o->flip_rel_torque);
+    s.worldContact.contactNormal = to_vec(o->world_contact_normal);
+    s.isOnGround = o->is_on_ground;
+    s.hasJumped = o->has_jumped;
+    s.hasDoubleJumped = o->has_double_jumped;
+    s.hasFlipped = o->has_flipped;
+    s.isFlipping = o->is_flipping;
+    s.isJumping = o->is_jumping;
+    s.isSupersonic = o->is_supersonic;
+    s.isAutoFlipping = o->is_auto_flipping;
+    s.isDemoed = o->is_demoed;
+    s.worldContact.hasContact = o->world_contact;
+    s.ballHitInfo.isValid = o->ball_hit_valid;
+    for (int i = 0; i < 4; i++) s.wheelsWithContact[i] = o->wheels_with_contact[i];
+    s.carContact.otherCarID = o->car_contact_other_id;
+    s.ballHitInfo.tickCountWhenHit = o->ball_hit_tick;
+    s.ballHitInfo.tickCountWhenExtraImpulseApplied = o->ball_hit_extra_impulse_tick;
+    s.ballHitInfo.relativePosOnBall = to_vec(o->ball_hit_rel_pos);
+    s.ballHitInfo.ballPos = to_vec(o->ball_hit_ball_pos);
+    s.ballHitInfo.extraHitVel = to_vec(o->ball_hit_extra_vel);
+    s.lastControls = to_controls(o->last_controls);
+    return s;
+}
+
+bool read_file(const std::string& path, std::vector<byte>& out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    return !out.empty();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_init(const char* mesh_dir) {
+    if (RocketSim::GetStage() == RocketSimStage::INITIALIZED) return 0;
+    std::map<GameMode, std::vector<FileData>> files;
+    for (int i = 0; i < 16; i++) {
+        std::vector<byte> data;
+        std::string path = std::string(mesh_dir) + "/mesh_" + std::to_string(i) + ".cmf";
+        if (!read_file(path, data)) return -1;
+        files[GameMode::SOCCAR].push_back(data);
+    }
+    RocketSim::InitFromMem(files, true);
+    return 0;
+}
+
+void* ref_arena_new(int n_blue, int n_orange) {
+    auto* h = new RefArena();
+    h->arena = Arena::Create(GameMode::SOCCAR);
+    for (int i = 0; i < n_blue; i++) h->cars.push_back(h->arena->AddCar(Team::BLUE, CAR_CONFIG_OCTANE));
+    for (int i = 0; i < n_orange; i++) h->cars.push_back(h->arena->AddCar(Team::ORANGE, CAR_CONFIG_OCTANE));
+    return h;
+}
+
+void ref_arena_free(void* p) {
+    auto* h = (RefArena*)p;
+    delete h->arena;
+    delete h;
+}
+
+int ref_num_cars(void* p) { return (int)((RefArena*)p)->cars.size(); }
+
+void ref_get_car(void* p, int i, rsx_car_state* out) {
+    auto* h = (RefArena*)p;
+    Car* car = h->cars[i];
+    to_public(car->GetState(), car, out);
+}
+
+void ref_set_car(void* p, int i, const rsx_car_state* in) {
+    auto* h = (RefArena*)p;
+    h->cars[i]->SetState(from_public(in));
+}
+
+void ref_set_controls(void* p, int i, const rsx_controls* c) {
+    auto* h = (RefArena*)p;
+    h->cars[i]->controls = to_controls(*c);
+}
+
+void ref_get_ball(void* p, rsx_ball_state* out) {
+    auto* h = (RefArena*)p;
+    BallState s = h->arena->ball->GetState();
+    out->pos = from_vec(s.pos);
+    out->vel = from_vec(s.vel);
+    out->ang_vel = from_vec(s.angVel);
+}
+
+void ref_set_ball(void* p, const rsx_ball_state* in) {
+    auto* h = (RefArena*)p;
+    BallState s;
+    s.pos = to_vec(in->pos);
+    s.vel = to_vec(in->vel);
+    s.angVel = to_vec(in->ang_vel);
+    h->arena->ball->SetState(s);
+}
+
+void ref_get_pads(void* p, rsx_pad_state* out) {
+    auto* h = (RefArena*)p;
+    auto& pads = h->arena->GetBoostPads();
+    for (size_t i = 0; i < pads.size(); i++) {
+        BoostPadState s = pads[i]->GetState();
+        memset(&out[i], 0, sizeof(rsx_pad_state));
+        out[i].is_active = s.isActive;
+        out[i].cooldown = s.cooldown;
+        out[i].prev_locked_car_id = s.prevLockedCarID;
+    }
+}
+
+void ref_set_pads(void* p, const rsx_pad_state* in) {
+    auto* h = (RefArena*)p;
+    auto& pads = h->arena->GetBoostPads();
+    for (size_t i = 0; i < pads.size(); i++) {
+        BoostPadState s;
+        s.isActive = in[i].is_active;
+        s.cooldown = in[i].cooldown;
+        s.curLockedCar = NULL;
+        s.prevLockedCarID = in[i].prev_locked_car_id;
+        pads[i]->SetState(s);
+    }
+}
+
+void ref_pad_config(void* p, float* xyz_big) {  // 34 x (x, y, z, isBig)
+    auto* h = (RefArena*)p;
+    auto& pads = h->arena->GetBoostPads();
+    for (size_t i = 0; i < pads.size(); i++) {
+        xyz_big[i * 4 + 0] = pads[i]->config.pos.x;
+        xyz_big[i * 4 + 1] = pads[i]->config.pos.y;
+        xyz_big[i * 4 + 2] = pads[i]->config.pos.z;
+        xyz_big[i * 4 + 3] = pads[i]->config.isBig ? 1.f : 0.f;
+    }
+}
+
+void ref_step(void* p, int ticks) { ((RefArena*)p)->arena->Step(ticks); }
+
+uint64_t ref_get_tick(void* p) { return ((RefArena*)p)->arena->tickCount; }
+void ref_set_tick(void* p, uint64_t t) { ((RefArena*)p)->arena->tickCount = t; }
+
+int ref_ball_scored(void* p) { return ((RefArena*)p)->arena->IsBallScored() ? 1 : 0; }
+
+void ref_reset_kickoff(void* p, int seed) { ((RefArena*)p)->arena->ResetToRandomKickoff(seed); }
+
+// Hidden per-wheel vehicle state (btWheelInfoRL, btVehicleRL.h) that carries
+// from one tick to the next: 4 wheels x 12 floats.
+void ref_get_wheels(void* p, int i, float* out) {
+    auto* h = (RefArena*)p;
+    Car* car = h->cars[i];
+    for (int w = 0; w < 4; w++) {
+        auto& wi = car->_bulletVehicle.m_wheelInfo[w];
+        float* o = out + w * 12;
+        o[0] = wi.m_steerAngle;
+        o[1] = wi.m_engineForce;
+        o[2] = wi.m_brake;
+        o[3] = wi.m_latFriction;
+        o[4] = wi.m_longFriction;
+        o[5] = wi.m_raycastInfo.m_suspensionLength;
+        o[6] = wi.m_suspensionRelativeVelocity;
+        o[7] = wi.m_clippedInvContactDotSuspension;
+        o[8] = wi.m_extraPushback;
+        o[9] = wi.m_raycastInfo.m_isInContact ? 1.f : 0.f;
+        o[10] = wi.m_wheelsSuspensionForce;
+        o[11] = wi.m_isInContactWithWorld ? 1.f : 0.f;
+    }
+}
+
+void ref_set_wheels(void* p, int i, const float* in) {
+    auto* h = (RefArena*)p;
+    Car* car = h->cars[i];
+    for (int w = 0; w < 4; w++) {
+        auto& wi = car->_bulletVehicle.m_wheelInfo[w];
+        const float* o = in + w * 12;
+        wi.m_steerAngle = o[0];
+        wi.m_engineForce = o[1];
+        wi.m_brake = o[2];
+        wi.m_latFriction = o[3];
+        wi.m_longFriction = o[4];
+    }
+}
+
+// Number of live contact points over all persistent manifolds of the arena's
+// Bullet world (a diagnostic for which scenario cases touch contacts at all).
+int ref_num_contacts(void* p) {
+    auto* h = (RefArena*)p;
+    auto* disp = h->arena->_bulletWorld.getDispatcher();
+    int n = 0;
+    for (int m = 0; m < disp->getNumManifolds(); m++) n += disp->getManifoldByIndexInternal(m)->getNumContacts();
+    return n;
+}
+
+uint32_t ref_sizeof_car_state(void) { return (uint32_t)sizeof(rsx_car_state); }
+
+// Identify a Bullet collision object of the arena: static bodies 0..N-1 in
+// creation order (meshes, then planes), the ball 100, car i 200+i.
+static int body_code(RefArena* h, const btCollisionObject* o) {
+    if (o == &h->arena->ball->_rigidBody) return 100;
+    for (size_t i = 0; i < h->cars.size(); i++)
+        if (o == &h->cars[i]->_rigidBody) return 200 + (int)i;
+    for (size_t i = 0; i < h->arena->_worldCollisionRBAmount; i++)
+        if (o == &h->arena->_worldCollisionRBs[i]) return (int)i;
+    return -1;
+}
+
+// Dump the manifolds Bullet built during the last tick, in dispatcher order.
+// Record per manifold: body0, body1, numPoints, breakingThreshold, then per
+// point 22 floats: posA(3) posB(3) normalB(3) dist friction restitution
+// isSpecial appliedImpulse localA(3) localB(3) lateralImpulse1 lifeTime.
+// Returns the number of floats written.
+int ref_dump_contacts(void* p, float* out, int max_floats) {
+    auto* h = (RefArena*)p;
+    auto* disp = h->arena->_bulletWorld.getDispatcher();
+    int n = 0;
+    for (int m = 0; m < disp->getNumManifolds(); m++) {
+        btPersistentManifold* mf = disp->getManifoldByIndexInternal(m);
+        int np = mf->getNumContacts();
+        if (n + 4 + np * 22 > max_floats) break;
+        out[n++] = (float)body_code(h, mf->getBody0());
+        out[n++] = (float)body_code(h, mf->getBody1());
+        out[n++] = (float)np;
+        out[n++] = mf->getContactBreakingThreshold();
+        for (int i = 0; i < np; i++) {
+            btManifoldPoint& cp = mf->getContactPoint(i);
+            const btVector3 vals[] = {cp.m_positionWorldOnA, cp.m_positionWorldOnB, cp.m_normalWorldOnB};
+            for (auto& v : vals) { out[n++] = v.x(); out[n++] = v.y(); out[n++] = v.z(); }
+            out[n++] = cp.m_distance1;
+            out[n++] = cp.m_combinedFriction;
+            out[n++] = cp.m_combinedRestitution;
+            out[n++] = cp.m_isSpecial ? 1.f : 0.f;
+            out[n++] = cp.m_appliedImpulse;
+            const btVector3 loc[] = {cp.m_localPointA, cp.m_localPointB};
+            for (auto& v : loc) { out[n++] = v.x(); out[n++] = v.y(); out[n++] = v.z(); }
+            out[n++] = cp.m_appliedImpulseLateral1;
+            out[n++] = (float)cp.m_lifeTime;
+        }
+    }
+    return n;
+}
+
+// World-space AABB (Bullet units) the broadphase holds for the ball (-1) or car i.
+void ref_get_aabb(void* p, int i, float* out6) {
+    auto* h = (RefArena*)p;
+    btCollisionObject* o = i < 0 ? (btCollisionObject*)&h->arena->ball->_rigidBody
+                                 : (btCollisionObject*)&h->cars[i]->_rigidBody;
+    btBroadphaseProxy* px = o->getBroadphaseHandle();
+    out6[0] = px->m_aabbMin.x(); out6[1] = px->m_aabbMin.y(); out6[2] = px->m_aabbMin.z();
+    out6[3] = px->m_aabbMax.x(); out6[4] = px->m_aabbMax.y(); out6[5] = px->m_aabbMax.z();
+}
+
+}  // extern "C"
