@@ -1,0 +1,353 @@
+// rsx_collide.h — contact generation of one tick, in the order and with the
+// per-point arithmetic of RocketSim's Bullet world.
+//
+// RocketSim's broadphase (btRSBroadphase.cpp:300-360) removes and re-adds every
+// overlapping pair each tick, which destroys and recreates every persistent
+// manifold; so the contact set of a tick depends only on that tick's state.
+// Pair order: dynamic bodies in creation order (ball, then cars), each paired
+// with the static bodies listed in its broadphase cell (meshes 0..15, then the
+// four planes, all AABB-tested), then with later dynamic bodies.
+//
+// Reference behaviour restated here:
+//   btManifoldResult::addContactPoint        btManifoldResult.cpp:120-207
+//   btPersistentManifold add/sort/refresh    btPersistentManifold.cpp:100-304
+//   btConvexPlaneCollisionAlgorithm          btConvexPlaneCollisionAlgorithm.cpp:105-165
+//   btConvexConcaveCollisionAlgorithm        btConvexConcaveCollisionAlgorithm.cpp:70-200
+//   SphereTriangleDetector                   SphereTriangleDetector.cpp:23-230
+//   btCompoundCollisionAlgorithm child AABB  btCompoundCollisionAlgorithm.cpp (ProcessChildShape)
+//   btGjkPairDetector result (sphere vs box) btGjkPairDetector.cpp (margins, normal on B)
+//   Arena::_BulletContactAddedCallback       Arena.cpp:218-427
+//   btCollisionWorld::updateSingleAabb       btCollisionWorld.cpp:143-176
+#pragma once
+
+#include "rsx_car.h"
+
+namespace rsx {
+
+constexpr int kBodyStatic = 100;  // body code base of static bodies
+
+struct TickCtx {
+    CarGeom cg;
+    BallGeom bg;
+    MeshView mv;
+    const PadGeom* pads;
+    float dt;
+};
+
+// --- manifold bookkeeping ----------------------------------------------------
+RSX_HD float calc_area4(V3 p0, V3 p1, V3 p2, V3 p3) {
+    V3 a0 = p0 - p1, b0 = p2 - p3;
+    V3 a1 = p0 - p2, b1 = p1 - p3;
+    V3 a2 = p0 - p3, b2 = p1 - p2;
+    V3 t0 = cross(a0, b0), t1 = cross(a1, b1), t2 = cross(a2, b2);
+    return maxf(maxf(len2(t0), len2(t1)), len2(t2));
+}
+
+// btPersistentManifold::sortCachedPoints (gContactCalcArea3Points = true)
+RSX_HD int sort_cached_points(const Manifold& m, const Contact& pt) {
+    int maxi = -1;
+    float maxpen = pt.dist;
+    for (int i = 0; i < 4; i++)
+        if (m.pt[i].dist < maxpen) {
+            maxi = i;
+            maxpen = m.pt[i].dist;
+        }
+    float r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    if (maxi != 0) r0 = len2(cross(pt.localA - m.pt[1].localA, m.pt[3].localA - m.pt[2].localA));
+    if (maxi != 1) r1 = len2(cross(pt.localA - m.pt[0].localA, m.pt[3].localA - m.pt[2].localA));
+    if (maxi != 2) r2 = len2(cross(pt.localA - m.pt[0].localA, m.pt[3].localA - m.pt[1].localA));
+    if (maxi != 3) r3 = len2(cross(pt.localA - m.pt[0].localA, m.pt[2].localA - m.pt[1].localA));
+    // btVector4::closestAxis4 = index of max |value|, first wins ties
+    float v[4] = {fabsf(r0), fabsf(r1), fabsf(r2), fabsf(r3)};
+    int best = -1;
+    float bv = -kLargeFloat;
+    for (int i = 0; i < 4; i++)
+        if (v[i] > bv) {
+            best = i;
+            bv = v[i];
+        }
+    return best;
+}
+
+// Add a point (normal on manifold body1 pointing to body0, point on body1,
+// signed distance); returns the index it landed at or -1 if rejected.
+RSX_HD int manifold_add(Manifold& m, const Xf& trA, const Xf& trB, V3 normal, V3 point_b, float depth, float friction,
+                        float restitution) {
+    if (depth > m.breaking) return -1;
+    V3 pa = point_b + normal * depth;
+    Contact c;
+    c.localA = inv_xform(trA, pa);
+    c.localB = inv_xform(trB, point_b);
+    c.normal = normal;
+    c.dist = depth;
+    c.posA = pa;
+    c.posB = point_b;
+    c.friction = friction;
+    c.restitution = restitution;
+    c.special = false;
+    int idx = m.n;
+    if (idx == kMaxManifoldPoints) {
+        idx = sort_cached_points(m, c);
+        if (idx < 0) idx = 0;
+    } else {
+        m.n++;
+    }
+    m.pt[idx] = c;
+    return idx;
+}
+
+RSX_HD void manifold_remove(Manifold& m, int i) {
+    int last = m.n - 1;
+    if (i != last) m.pt[i] = m.pt[last];
+    m.n--;
+}
+
+// btPersistentManifold::refreshContactPoints
+RSX_HD void manifold_refresh(Manifold& m, const Xf& trA, const Xf& trB) {
+    for (int i = m.n - 1; i >= 0; i--) {
+        Contact& c = m.pt[i];
+        c.posA = xform(trA, c.localA);
+        c.posB = xform(trB, c.localB);
+        c.dist = dot(c.posA - c.posB, c.normal);
+    }
+    for (int i = m.n - 1; i >= 0; i--) {
+        Contact& c = m.pt[i];
+        if (!(c.dist <= m.breaking)) {
+            manifold_remove(m, i);
+        } else {
+            V3 proj = c.posA - c.normal * c.dist;
+            V3 diff = c.posB - proj;
+            float d2 = dot(diff, diff);
+            if (d2 > m.breaking * m.breaking) manifold_remove(m, i);
+        }
+    }
+}
+
+// --- manifold list of the tick ------------------------------------------------
+constexpr int kMaxManifolds = 40;
+constexpr int kMaxLive = 12;
+
+struct MfList {
+    int count;
+    int8_t body0[kMaxManifolds], body1[kMaxManifolds];  // 0 ball, 1+i car i, -1 static
+    int8_t slot[kMaxManifolds];                          // index into live, -1 empty
+    int nlive;
+    Manifold live[kMaxLive];
+};
+
+RSX_HD int mf_open(MfList& L, int b0, int b1, float breaking) {
+    int k = L.count++;
+    L.body0[k] = (int8_t)b0;
+    L.body1[k] = (int8_t)b1;
+    L.slot[k] = -1;
+    if (L.nlive < kMaxLive) {
+        int s = L.nlive;
+        Manifold& m = L.live[s];
+        m.body0 = b0;
+        m.body1 = b1;
+        m.n = 0;
+        m.breaking = breaking;
+        L.slot[k] = (int8_t)s;
+    }
+    return k;
+}
+// Keep the manifold's storage only if it ended the tick with contacts.
+RSX_HD void mf_close(MfList& L, int k) {
+    int s = L.slot[k];
+    if (s < 0) return;
+    if (L.live[s].n > 0 && s == L.nlive) L.nlive++;
+    else if (L.live[s].n == 0) L.slot[k] = -1;
+}
+
+// --- AABBs (btCollisionWorld::updateSingleAabb) --------------------------------
+RSX_HD void ball_shape_aabb(const BallGeom& bg, V3 c, V3& mn, V3& mx) {
+    float m = bg.radius + 0.08f;
+    V3 e = V3{m, m, m};
+    mn = c - e;
+    mx = c + e;
+}
+RSX_HD V3 abs_dot3(V3 h, const M3& b) {
+    M3 a = M3{{vabs(b.r[0]), vabs(b.r[1]), vabs(b.r[2])}};
+    return V3{dot(h, a.r[0]), dot(h, a.r[1]), dot(h, a.r[2])};
+}
+RSX_HD void car_shape_aabb(const CarGeom& g, const Xf& t, V3& mn, V3& mx) {
+    V3 half = g.compound_half;  // compound margin is 0
+    V3 center = xform(t, g.compound_center);
+    V3 ext = abs_dot3(half, t.b);
+    mn = center - ext;
+    mx = center + ext;
+}
+RSX_HD void box_child_aabb(const CarGeom& g, const Xf& child, V3& mn, V3& mx) {
+    V3 h = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+    V3 ext = abs_dot3(h, child.b);
+    mn = child.o - ext;
+    mx = child.o + ext;
+}
+RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
+    return Xf{car.b * m3_identity(), xform(car, g.box_offset)};
+}
+
+RSX_HD void broad_aabb(V3 mn1, V3 mx1, V3 mn2, V3 mx2, V3& mn, V3& mx) {
+    const float t = kContactBreakingThreshold;
+    V3 e = V3{t, t, t};
+    mn = vmin(mn1 - e, mn2 - e);
+    mx = vmax(mx1 + e, mx2 + e);
+}
+
+// Plane shape AABB (btStaticPlaneShape::getAabb with the RocketSim slab).
+RSX_HD void plane_shape_aabb(int p, V3& mn, V3& mx) {
+    const float L = kLargeFloat, off = 0.2f;
+    mn = V3{-L, -L, -L};
+    mx = V3{L, L, L};
+    V3 o = plane_origin(p);
+    if (p == 0) { mn.z = o.z + (0 - off); mx.z = o.z + (0 + off); mn.z = -L; }
+    if (p == 1) { mn.z = o.z + (0 - off); mx.z = o.z + (0 + off); mx.z = L; }
+    if (p == 2) { mn.x = o.x + (0 - off); mx.x = o.x + (0 + off); mn.x = -L; }
+    if (p == 3) { mn.x = o.x + (0 - off); mx.x = o.x + (0 + off); mx.x = L; }
+}
+
+RSX_HD int bp_cell_index(const MeshView& w, V3 pos) {
+    V3 f = (pos - V3{kBpMinX, kBpMinY, kBpMinZ}) / kBpCellSize;
+    int i = (int)f.x, j = (int)f.y, k = (int)f.z;
+    i = i < 0 ? 0 : (i > w.bp_nx - 1 ? w.bp_nx - 1 : i);
+    j = j < 0 ? 0 : (j > w.bp_ny - 1 ? w.bp_ny - 1 : j);
+    k = k < 0 ? 0 : (k > w.bp_nz - 1 ? w.bp_nz - 1 : k);
+    return i * w.bp_ny * w.bp_nz + j * w.bp_nz + k;
+}
+
+// --- sphere vs plane / triangle ------------------------------------------------
+// btConvexPlaneCollisionAlgorithm for the ball (convex) against plane p.
+RSX_HD bool sphere_plane(const BallGeom& bg, V3 center, int p, float breaking, V3& normal, V3& pt, float& dist) {
+    V3 pn = plane_normal(p);
+    Xf pt_xf = Xf{m3_identity(), plane_origin(p)};
+    Xf ball_xf = Xf{m3_identity(), center};
+    Xf plane_in_convex = xf_mul(xf_inverse(ball_xf), pt_xf);
+    Xf convex_in_plane = xf_mul(xf_inverse(pt_xf), ball_xf);
+    V3 dir = plane_in_convex.b * (-pn);
+    // btConvexInternalShape::localGetSupportingVertex on a sphere: 0 + margin*normalize(dir)
+    V3 dn = dir;
+    if (len2(dn) < kEps * kEps) dn = V3{-1, -1, -1};
+    dn = normalized(dn);
+    V3 vtx = V3{0, 0, 0} + dn * bg.radius;
+    V3 vip = xform(convex_in_plane, vtx);
+    dist = dot(pn, vip) - 0.f;
+    if (!(dist < breaking)) return false;
+    V3 proj = vip - pn * dist;
+    pt = xform(pt_xf, proj);
+    normal = pt_xf.b * pn;
+    return true;
+}
+
+// closestPointTriangle (SphereTriangleDetector.cpp, "faster method")
+RSX_HD V3 closest_point_triangle(V3 p, V3 a, V3 b, V3 c) {
+    const V3 ab = b - a, ac = c - a, ap = p - a;
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.f && d2 <= 0.f) return a;
+    const V3 bp = p - b;
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.f && d4 <= d3) return b;
+    const V3 cp = p - c;
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.f && d5 <= d6) return c;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+        const float v = d1 / (d1 - d3);
+        return a + v * ab;
+    }
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+        const float v = d2 / (d2 - d6);
+        return a + v * ac;
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+        const float v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return b + v * (c - b);
+    }
+    const float denom = 1.f / (va + vb + vc);
+    const float v = vb * denom;
+    const float w = vc * denom;
+    return a + v * ab + w * ac;
+}
+
+RSX_HD bool point_in_triangle(V3 p0, V3 p1, V3 p2, V3 p) {
+    V3 u = p1 - p0, v = p2 - p0;
+    V3 n = cross(u, v);
+    float nl = dot(n, n);
+    V3 w = p - p0;
+    float gamma = dot(cross(u, w), n) / nl;
+    float beta = dot(cross(w, v), n) / nl;
+    float alpha = 1 - gamma - beta;
+    return (0 <= alpha) && (alpha <= 1) && (0 <= beta) && (beta <= 1) && (0 <= gamma) && (gamma <= 1);
+}
+
+// SphereTriangleDetector::collide (sphere centre in triangle space = world)
+RSX_HD bool sphere_triangle(V3 center, float radius, float threshold, V3 v0, V3 v1, V3 v2, V3& point, V3& rnormal,
+                            float& depth) {
+    float rwt = radius + threshold;
+    V3 normal = cross(v1 - v0, v2 - v0);
+    float l2 = len2(normal);
+    bool has = false;
+    V3 cpnt;
+    if (l2 >= kEps * kEps) {
+        normal = normal / sqrtf(l2);
+        V3 p1c = center - v0;
+        float dfp = dot(p1c, normal);
+        if (dfp < 0.f) {
+            dfp *= -1.f;
+            normal = normal * -1.f;
+        }
+        if (dfp < rwt) {
+            if (point_in_triangle(v0, v1, v2, center)) {
+                has = true;
+                cpnt = center - normal * dfp;
+            } else {
+                float min_d2 = rwt * rwt;
+                V3 ne = closest_point_triangle(center, v0, v1, v2);
+                float d2 = dist2(ne, center);
+                if (d2 < min_d2) {
+                    has = true;
+                    cpnt = ne;
+                }
+            }
+        }
+    }
+    if (has) {
+        V3 c2c = center - cpnt;
+        float d2 = len2(c2c);
+        if (d2 < rwt * rwt) {
+            if (d2 > kEps) {
+                float d = sqrtf(d2);
+                rnormal = normalized(c2c);
+                point = cpnt;
+                depth = -(radius - d);
+            } else {
+                rnormal = normal;
+                point = cpnt;
+                depth = -radius;
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// btConvexTriangleCallback early-out for a sphere (support points along
+// +-triangle normal): true if the triangle can be skipped.
+RSX_HD bool sphere_tri_early_out(V3 center, float radius, V3 v0, V3 v1, V3 v2, float threshold) {
+    V3 tn = normalized(cross(v1 - v0, v2 - v0));
+    for (int s = 0; s < 2; s++) {
+        V3 dir = tn;  // ball basis is identity: inverse() * n == n
+        V3 dn = dir;
+        if (len2(dn) < kEps * kEps) dn = V3{-1, -1, -1};
+        dn = normalized(dn);
+        V3 world = center + dn * radius;
+        float pd = dot(tn, world);
+        float pt = dot(tn, v0);
+        if (pt - pd > threshold) return true;
+        tn = tn * -1.f;
+    }
+    return false;
+}
+
+}  // namespace rsx

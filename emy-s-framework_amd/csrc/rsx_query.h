@@ -1,0 +1,197 @@
+// rsx_query.h — geometric queries against the soccar arena: the static
+// planes, the triangle mesh (uniform-grid index, visited in the order of the
+// reference's per-mesh tree so that contact insertion order matches), and the
+// suspension collision grid's fast path.
+//
+// Reference behaviour restated here:
+//   SuspensionCollisionGrid::CastSuspensionRay   SuspensionCollisionGrid.cpp:124-175
+//   btTriangleRaycastCallback::processTriangle   btRaycastCallback.cpp:35-128
+//   btCollisionWorld::ClosestRayResultCallback   btCollisionWorld.h:211-250
+//   btDefaultVehicleRaycaster::castRay           btDefaultVehicleRaycaster.cpp:32-53
+//   TestTriangleAgainstAabb2 / TestAabbAgainstAabb2  btAabbUtil2.h:43-70
+#pragma once
+
+#include "rsx_world.h"
+
+namespace rsx {
+
+// Static planes of the soccar arena (Arena.cpp:1060-1101), Bullet units.
+RSX_HD V3 plane_normal(int p) {
+    return p == 0 ? V3{0, 0, 1} : (p == 1 ? V3{0, 0, -1} : (p == 2 ? V3{1, 0, 0} : V3{-1, 0, 0}));
+}
+RSX_HD V3 plane_origin(int p) {
+    return p == 0 ? V3{0, 0, 0}
+                  : (p == 1 ? V3{0, 0, kArenaHeight * kUU2BT}
+                            : (p == 2 ? V3{-kArenaExtentX * kUU2BT, 0, kArenaHeight / 2 * kUU2BT}
+                                      : V3{kArenaExtentX * kUU2BT, 0, kArenaHeight / 2 * kUU2BT}));
+}
+
+RSX_HD bool aabb_overlap(V3 mn1, V3 mx1, V3 mn2, V3 mx2) {
+    bool o = true;
+    o = (mn1.x > mx2.x || mx1.x < mn2.x) ? false : o;
+    o = (mn1.z > mx2.z || mx1.z < mn2.z) ? false : o;
+    o = (mn1.y > mx2.y || mx1.y < mn2.y) ? false : o;
+    return o;
+}
+
+RSX_HD bool tri_aabb_overlap(V3 a, V3 b, V3 c, V3 mn, V3 mx) {
+    // z, then x, then y (btAabbUtil2.h:54-70)
+    if (minf(minf(a.z, b.z), c.z) > mx.z) return false;
+    if (maxf(maxf(a.z, b.z), c.z) < mn.z) return false;
+    if (minf(minf(a.x, b.x), c.x) > mx.x) return false;
+    if (maxf(maxf(a.x, b.x), c.x) < mn.x) return false;
+    if (minf(minf(a.y, b.y), c.y) > mx.y) return false;
+    if (maxf(maxf(a.y, b.y), c.y) < mn.y) return false;
+    return true;
+}
+
+RSX_HD void load_tri(const MeshView& w, int t, V3& a, V3& b, V3& c) {
+    const float* p = w.tri + (size_t)t * 9;
+    a = V3{p[0], p[1], p[2]};
+    b = V3{p[3], p[4], p[5]};
+    c = V3{p[6], p[7], p[8]};
+}
+
+RSX_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Candidate triangles overlapping [mn, mx], de-duplicated, sorted by
+// (mesh index, tree traversal rank).  Returns the count (capped at cap).
+RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
+    const float inv = 1.0f / w.grid_cell;
+    int x0 = clampi((int)floorf((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    int y0 = clampi((int)floorf((mn.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    int z0 = clampi((int)floorf((mn.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    int x1 = clampi((int)floorf((mx.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    int y1 = clampi((int)floorf((mx.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    int z1 = clampi((int)floorf((mx.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    int n = 0;
+    for (int ix = x0; ix <= x1; ix++)
+        for (int iy = y0; iy <= y1; iy++)
+            for (int iz = z0; iz <= z1; iz++) {
+                int cell = (ix * w.gy + iy) * w.gz + iz;
+                int s = w.cell_start[cell], e = w.cell_start[cell + 1];
+                for (int k = s; k < e; k++) {
+                    int t = w.cell_tris[k];
+                    // de-duplicate: keep only if not already present
+                    bool dup = false;
+                    for (int j = 0; j < n; j++)
+                        if (out[j] == t) { dup = true; break; }
+                    if (dup) continue;
+                    V3 a, b, c;
+                    load_tri(w, t, a, b, c);
+                    if (!tri_aabb_overlap(a, b, c, mn, mx)) continue;
+                    if (n < cap) out[n++] = t;
+                }
+            }
+    // insertion sort by (mesh, rank)
+    for (int i = 1; i < n; i++) {
+        int t = out[i];
+        int key = w.tri_mesh[t] * 65536 + w.tri_rank[t];
+        int j = i - 1;
+        while (j >= 0 && (w.tri_mesh[out[j]] * 65536 + w.tri_rank[out[j]]) > key) {
+            out[j + 1] = out[j];
+            j--;
+        }
+        out[j + 1] = t;
+    }
+    return n;
+}
+
+struct RayHit {
+    V3 point, normal;
+    float fraction;
+    int object;     // -1 none, 0 static world, 1 + b dynamic body b (0 ball, 1+i car i)
+};
+
+// btTriangleRaycastCallback::processTriangle on one triangle; updates hit if closer.
+RSX_HD bool ray_triangle(V3 from, V3 to, V3 v0, V3 v1, V3 v2, float& hit_fraction, V3& hit_normal) {
+    V3 v10 = v1 - v0;
+    V3 v20 = v2 - v0;
+    V3 tn = cross(v10, v20);
+    const float d = dot(v0, tn);
+    float da = dot(tn, from);
+    da -= d;
+    float db = dot(tn, to);
+    db -= d;
+    if (da * db >= 0.0f) return false;
+    const float proj = da - db;
+    const float distance = da / proj;
+    if (!(distance < hit_fraction)) return false;
+    float tol = len2(tn);
+    tol *= -0.0001f;
+    float s = 1.0f - distance;
+    V3 point = V3{s * from.x + distance * to.x, s * from.y + distance * to.y, s * from.z + distance * to.z};
+    V3 v0p = v0 - point;
+    V3 v1p = v1 - point;
+    V3 cp0 = cross(v0p, v1p);
+    if (!(dot(cp0, tn) >= tol)) return false;
+    V3 v2p = v2 - point;
+    V3 cp1 = cross(v1p, v2p);
+    if (!(dot(cp1, tn) >= tol)) return false;
+    V3 cp2 = cross(v2p, v0p);
+    if (!(dot(cp2, tn) >= tol)) return false;
+    tn = normalized(tn);
+    hit_normal = (da <= 0.0f) ? -tn : tn;
+    hit_fraction = distance;
+    return true;
+}
+
+// Full Bullet ray test against the static world (mesh triangles near the ray,
+// then the four planes), closest hit wins (strictly closer replaces).
+RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit) {
+    V3 mn = vmin(from, to), mx = vmax(from, to);
+    int cand[48];
+    int n = gather_tris(w, mn, mx, cand, 48);
+    float frac = hit.object >= 0 ? hit.fraction : 1.0f;
+    for (int i = 0; i < n; i++) {
+        V3 a, b, c;
+        load_tri(w, cand[i], a, b, c);
+        V3 nrm;
+        if (ray_triangle(from, to, a, b, c, frac, nrm)) {
+            hit.fraction = frac;
+            hit.normal = nrm;
+            hit.object = 0;
+        }
+    }
+    for (int p = 0; p < kNumPlanes; p++) {
+        // btStaticPlaneShape::processAllTriangles around the ray's local AABB,
+        // then the triangle ray test on its two triangles.
+        V3 pn = plane_normal(p);
+        V3 po = plane_origin(p);
+        Xf inv = xf_inverse(Xf{m3_identity(), po});
+        V3 lf = xform(inv, from), lt = xform(inv, to);
+        V3 amin = vmin(lf, lt), amax = vmax(lf, lt);
+        V3 half = (amax - amin) * 0.5f;
+        float radius = len(half);
+        V3 center = (amax + amin) * 0.5f;
+        V3 t0, t1;
+        plane_space(pn, t0, t1);
+        V3 pc = center - (dot(pn, center) - 0.f) * pn;
+        V3 ta = pc + t0 * radius + t1 * radius;
+        V3 tb = pc + t0 * radius - t1 * radius;
+        V3 tc = pc - t0 * radius - t1 * radius;
+        V3 nrm;
+        if (ray_triangle(lf, lt, ta, tb, tc, frac, nrm)) {
+            hit.fraction = frac;
+            hit.normal = m3_identity() * nrm;
+            hit.object = 0;
+        }
+        V3 ua = pc - t0 * radius - t1 * radius;
+        V3 ub = pc - t0 * radius + t1 * radius;
+        V3 uc = pc + t0 * radius + t1 * radius;
+        if (ray_triangle(lf, lt, ua, ub, uc, frac, nrm)) {
+            hit.fraction = frac;
+            hit.normal = m3_identity() * nrm;
+            hit.object = 0;
+        }
+    }
+}
+
+RSX_HD void susp_cell_indices(V3 pos_uu, int& i, int& j, int& k) {
+    i = (int)clampf(pos_uu.x / kSuspCellSizeX + (kSuspCellsX / 2), 0, kSuspCellsX - 1);
+    j = (int)clampf(pos_uu.y / kSuspCellSizeY + (kSuspCellsY / 2), 0, kSuspCellsY - 1);
+    k = (int)clampf(pos_uu.z / kSuspCellSizeZ, 0, kSuspCellsZ - 1);
+}
+RSX_HD int susp_cell(int i, int j, int k) { return (i * kSuspCellsY * kSuspCellsZ) + (j * kSuspCellsZ) + k; }
+
+}  // namespace rsx

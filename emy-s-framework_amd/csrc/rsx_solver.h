@@ -1,0 +1,312 @@
+// rsx_solver.h — the sequential-impulse contact solver of one tick, with the
+// RocketSim modifications to Bullet 3.24:
+//   * split impulse always on (penetration threshold 1e30, erp2 0.8),
+//   * no positional term folded into the velocity rhs,
+//   * ball-world points marked "special": their normal rows only take part in
+//     the split-impulse (position) pass, and one averaged row per body carries
+//     the velocity response.
+// Reference: btSequentialImpulseConstraintSolver.cpp:45-105 (row solvers),
+// :283-313 (split impulse row), :458-491 (initSolverBody), :517-606
+// (friction rows), :795-983 (contact rows), :1003-1211 (convertContact,
+// convertContactInner, convertContactSpecial), :1422-1599 (setup),
+// :1601-1829 (iterations), :1831-1927 (write-back); btSolverBody.h:130-300.
+#pragma once
+
+#include "rsx_collide.h"
+
+namespace rsx {
+
+struct SBody {
+    V3 dlin, dang, push, turn;
+    V3 lin, ang, ext_f, ext_t;
+    float inv_mass;
+    M3 inv_i;
+    Xf xf;
+    bool dyn;        // has an original rigid body with mass
+    bool used;
+};
+
+struct Row {
+    V3 n1, rc1, n2, rc2, angA, angB;
+    float applied, push, rhs, rhs_pen, jac_inv, lo, hi, friction;
+    int8_t a, b;
+    bool special;
+    int16_t fidx;
+};
+
+constexpr int kMaxRows = 40;
+
+template <int MAXC>
+struct Solver {
+    SBody body[MAXC + 2];  // 0 ball, 1.. cars, MAXC+1 fixed
+    Row row[kMaxRows];
+    Row fric[kMaxRows];
+    int nrow, nfric;
+    // per-body special accumulation (btCollisionObject::m_specialResolveInfo)
+    int nspecial[MAXC + 1];
+    float sp_friction[MAXC + 1], sp_restitution[MAXC + 1], sp_dist[MAXC + 1];
+    V3 sp_normal[MAXC + 1];
+};
+
+RSX_HD void sbody_init_fixed(SBody& s) {
+    V3 z = V3{0, 0, 0};
+    s.dlin = s.dang = s.push = s.turn = z;
+    s.lin = s.ang = s.ext_f = s.ext_t = z;
+    s.inv_mass = 0;
+    s.inv_i = M3{{z, z, z}};
+    s.xf = xf_identity();
+    s.dyn = false;
+    s.used = true;
+}
+
+RSX_HD void sbody_init(SBody& s, const Xf& xf, V3 lin, V3 ang, V3 force, V3 torque, float inv_mass, const M3& inv_i,
+                       float dt) {
+    V3 z = V3{0, 0, 0};
+    s.dlin = s.dang = s.push = s.turn = z;
+    s.xf = xf;
+    s.inv_mass = inv_mass;
+    s.inv_i = inv_i;
+    s.lin = lin;
+    s.ang = ang;
+    s.ext_f = force * inv_mass * dt;
+    s.ext_t = torque * inv_i * dt;
+    s.dyn = true;
+    s.used = true;
+}
+
+RSX_HD float restitution_curve(float rel_vel, float restitution) {
+    if (fabsf(rel_vel) < kRestitutionVelThreshold) return 0.f;
+    return restitution * -rel_vel;
+}
+
+template <int MAXC>
+RSX_HD void setup_contact_row(Solver<MAXC>& S, Row& r, int ia, int ib, V3 normal, float dist, float friction,
+                              float restitution, bool special, V3 rel1, V3 rel2, float dt) {
+    SBody& A = S.body[ia];
+    SBody& B = S.body[ib];
+    r.a = (int8_t)ia;
+    r.b = (int8_t)ib;
+    const float relaxation = 1.f;
+    float inv_dt = 1.f / dt;
+    const float erp = kErp2;
+    V3 ta = cross(rel1, normal);
+    r.angA = A.dyn ? (A.inv_i * ta) : V3{0, 0, 0};
+    V3 tb = cross(rel2, normal);
+    r.angB = B.dyn ? (B.inv_i * (-tb)) : V3{0, 0, 0};
+    float d0 = 0.f, d1 = 0.f;
+    if (A.dyn) d0 = A.inv_mass + dot(normal, cross(r.angA, rel1));
+    if (B.dyn) d1 = B.inv_mass + dot(normal, cross(-r.angB, rel2));
+    r.jac_inv = relaxation / (d0 + d1 + 0.f);
+    if (A.dyn) {
+        r.n1 = normal;
+        r.rc1 = ta;
+    } else {
+        r.n1 = r.rc1 = V3{0, 0, 0};
+    }
+    if (B.dyn) {
+        r.n2 = -normal;
+        r.rc2 = -tb;
+    } else {
+        r.n2 = r.rc2 = V3{0, 0, 0};
+    }
+    float penetration = dist + 0.f;
+    {
+        V3 v1 = A.dyn ? (A.lin + cross(A.ang, rel1)) : V3{0, 0, 0};
+        V3 v2 = B.dyn ? (B.lin + cross(B.ang, rel2)) : V3{0, 0, 0};
+        float rv = dot(normal, v1 - v2);
+        r.friction = friction;
+        float rest = restitution_curve(rv, restitution);
+        restitution = rest <= 0.f ? 0.f : rest;
+    }
+    r.applied = 0.f;  // warm start: fresh manifolds carry no impulse
+    r.push = 0.f;
+    {
+        V3 efa = A.dyn ? A.ext_f : V3{0, 0, 0};
+        V3 eta = A.dyn ? A.ext_t : V3{0, 0, 0};
+        V3 efb = B.dyn ? B.ext_f : V3{0, 0, 0};
+        V3 etb = B.dyn ? B.ext_t : V3{0, 0, 0};
+        float v1 = dot(r.n1, A.lin + efa) + dot(r.rc1, A.ang + eta);
+        float v2 = dot(r.n2, B.lin + efb) + dot(r.rc2, B.ang + etb);
+        float rv = v1 + v2;
+        float pos_err = 0.f;
+        float vel_err = restitution - rv;
+        if (penetration > 0) pos_err = 0;
+        else pos_err = -penetration * erp * inv_dt;
+        r.rhs = vel_err * r.jac_inv;
+        r.rhs_pen = pos_err * r.jac_inv;
+        r.lo = 0;
+        r.hi = 1e10f;
+    }
+    r.special = special;
+}
+
+template <int MAXC>
+RSX_HD void add_friction_row(Solver<MAXC>& S, V3 dir, int ia, int ib, float friction, V3 rel1, V3 rel2, int fidx) {
+    if (S.nfric >= kMaxRows) return;
+    Row& r = S.fric[S.nfric++];
+    SBody& A = S.body[ia];
+    SBody& B = S.body[ib];
+    r.a = (int8_t)ia;
+    r.b = (int8_t)ib;
+    r.fidx = (int16_t)fidx;
+    r.friction = friction;
+    r.applied = 0.f;
+    r.push = 0.f;
+    r.special = false;
+    if (A.dyn) {
+        r.n1 = dir;
+        r.rc1 = cross(rel1, r.n1);
+        r.angA = A.inv_i * r.rc1;
+    } else {
+        r.n1 = r.rc1 = r.angA = V3{0, 0, 0};
+    }
+    if (B.dyn) {
+        r.n2 = -dir;
+        r.rc2 = cross(rel2, r.n2);
+        r.angB = B.inv_i * r.rc2;
+    } else {
+        r.n2 = r.rc2 = r.angB = V3{0, 0, 0};
+    }
+    float d0 = 0.f, d1 = 0.f;
+    if (A.dyn) d0 = A.inv_mass + dot(dir, cross(r.angA, rel1));
+    if (B.dyn) d1 = B.inv_mass + dot(dir, cross(-r.angB, rel2));
+    r.jac_inv = 1.f / (d0 + d1);
+    float v1 = dot(r.n1, A.dyn ? A.lin + A.ext_f : V3{0, 0, 0}) + dot(r.rc1, A.dyn ? A.ang : V3{0, 0, 0});
+    float v2 = dot(r.n2, B.dyn ? B.lin + B.ext_f : V3{0, 0, 0}) + dot(r.rc2, B.dyn ? B.ang : V3{0, 0, 0});
+    float rv = v1 + v2;
+    float vel_err = 0.f - rv;
+    r.rhs = 0.f + vel_err * r.jac_inv;
+    r.rhs_pen = 0.f;
+    r.lo = -friction;
+    r.hi = friction;
+}
+
+// convertContactInner without rolling friction (all rolling frictions are 0)
+template <int MAXC>
+RSX_HD void add_contact_friction(Solver<MAXC>& S, int ia, int ib, V3 normal, float friction, V3 rel1, V3 rel2,
+                                 int fidx) {
+    SBody& A = S.body[ia];
+    SBody& B = S.body[ib];
+    V3 v1 = A.dyn ? (A.lin + A.ext_f + cross(A.ang + A.ext_t, rel1)) : V3{0, 0, 0};
+    V3 v2 = B.dyn ? (B.lin + B.ext_f + cross(B.ang + B.ext_t, rel2)) : V3{0, 0, 0};
+    V3 vel = v1 - v2;
+    float rv = dot(normal, vel);
+    V3 lat = vel - normal * rv;
+    float l2 = len2(lat);
+    if (l2 > kEps) {
+        lat *= 1.f / sqrtf(l2);
+        add_friction_row(S, lat, ia, ib, friction, rel1, rel2, fidx);
+    } else {
+        V3 p, q;
+        plane_space(normal, p, q);
+        add_friction_row(S, p, ia, ib, friction, rel1, rel2, fidx);
+    }
+}
+
+RSX_HD void sbody_apply(SBody& s, V3 lin_comp, V3 ang_comp, float imp) {
+    if (s.dyn) {
+        s.dlin += lin_comp * imp;
+        s.dang += ang_comp * imp;
+    }
+}
+RSX_HD void sbody_apply_push(SBody& s, V3 lin_comp, V3 ang_comp, float imp) {
+    if (s.dyn) {
+        s.push += lin_comp * imp;
+        s.turn += ang_comp * imp;
+    }
+}
+
+template <int MAXC>
+RSX_HD float solve_split(Solver<MAXC>& S, Row& c) {
+    float delta = 0.f;
+    if (c.rhs_pen != 0.f) {
+        SBody& A = S.body[c.a];
+        SBody& B = S.body[c.b];
+        delta = c.rhs_pen - c.push * 0.f;
+        const float dv1 = dot(c.n1, A.push) + dot(c.rc1, A.turn);
+        const float dv2 = dot(c.n2, B.push) + dot(c.rc2, B.turn);
+        delta -= dv1 * c.jac_inv;
+        delta -= dv2 * c.jac_inv;
+        const float sum = c.push + delta;
+        if (sum < c.lo) {
+            delta = c.lo - c.push;
+            c.push = c.lo;
+        } else {
+            c.push = sum;
+        }
+        sbody_apply_push(A, c.n1 * A.inv_mass, c.angA, delta);
+        sbody_apply_push(B, c.n2 * B.inv_mass, c.angB, delta);
+    }
+    return (float)((double)delta * (1. / (double)c.jac_inv));
+}
+
+template <int MAXC>
+RSX_HD void solve_row(Solver<MAXC>& S, Row& c, bool upper) {
+    SBody& A = S.body[c.a];
+    SBody& B = S.body[c.b];
+    float delta = c.rhs - c.applied * 0.f;
+    const float dv1 = dot(c.n1, A.dlin) + dot(c.rc1, A.dang);
+    const float dv2 = dot(c.n2, B.dlin) + dot(c.rc2, B.dang);
+    delta -= dv1 * c.jac_inv;
+    delta -= dv2 * c.jac_inv;
+    const float sum = c.applied + delta;
+    if (sum < c.lo) {
+        delta = c.lo - c.applied;
+        c.applied = c.lo;
+    } else if (upper && sum > c.hi) {
+        delta = c.hi - c.applied;
+        c.applied = c.hi;
+    } else {
+        c.applied = sum;
+    }
+    sbody_apply(A, c.n1 * A.inv_mass, c.angA, delta);
+    sbody_apply(B, c.n2 * B.inv_mass, c.angB, delta);
+}
+
+template <int MAXC>
+RSX_HD void solver_iterate(Solver<MAXC>& S) {
+    for (int it = 0; it < kSolverIterations; it++) {
+        float res = 0.f;
+        for (int j = 0; j < S.nrow; j++) {
+            float r = solve_split(S, S.row[j]);
+            res = maxf(res, r * r);
+        }
+        if (res <= 0.f || it >= (kSolverIterations - 1)) break;
+    }
+    for (int it = 0; it < kSolverIterations; it++) {
+        for (int j = 0; j < S.nrow; j++) {
+            if (S.row[j].special) continue;
+            solve_row(S, S.row[j], false);
+        }
+        for (int j = 0; j < S.nfric; j++) {
+            Row& f = S.fric[j];
+            float total = S.row[f.fidx].applied;
+            if (total > 0.f) {
+                f.lo = -(f.friction * total);
+                f.hi = f.friction * total;
+                solve_row(S, f, true);
+            }
+        }
+    }
+}
+
+// btSolverBody::writebackVelocityAndTransform + writeBackBodies
+RSX_HD void sbody_writeback(SBody& s, bool no_rot, float dt, Xf& xf_out, V3& lin_out, V3& ang_out) {
+    s.lin += s.dlin;
+    s.ang += s.dang;
+    if (s.push.x != 0.f || s.push.y != 0 || s.push.z != 0 || s.turn.x != 0.f || s.turn.y != 0 || s.turn.z != 0) {
+        if (no_rot) {
+            Xf n;
+            n.o = s.xf.o + s.push * dt;
+            n.b = s.xf.b;
+            s.xf = n;
+        } else {
+            s.xf = integrate_transform(s.xf, s.push, s.turn * kSplitImpulseTurnErp, dt);
+        }
+    }
+    lin_out = s.lin + s.ext_f;
+    ang_out = s.ang + s.ext_t;
+    xf_out = s.xf;
+}
+
+}  // namespace rsx

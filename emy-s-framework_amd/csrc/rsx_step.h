@@ -1,0 +1,657 @@
+// rsx_step.h — one physics tick of one arena: the order of Arena::Step
+// (Arena.cpp:716-812) with Bullet's world step in the middle
+// (btDiscreteDynamicsWorld.cpp:343-428: gravity, damping + prediction,
+// broadphase + narrowphase, islands, solve, integrate), then car post-tick,
+// boost pads, ball limits and the goal test.
+#pragma once
+
+#include "rsx_solver.h"
+
+namespace rsx {
+
+// ---- Bullet's unstable quicksort (btAlignedObjectArray::quickSortInternal) ----
+RSX_HD void bt_quicksort(int* key, int* val, int lo, int hi) {
+    // explicit stack: ranges are tiny (<= kMaxManifolds)
+    int stk_lo[64], stk_hi[64];
+    int sp = 0;
+    stk_lo[sp] = lo;
+    stk_hi[sp] = hi;
+    sp++;
+    while (sp > 0) {
+        sp--;
+        int L = stk_lo[sp], H = stk_hi[sp];
+        int i = L, j = H;
+        int x = key[(L + H) / 2];
+        do {
+            while (key[i] < x) i++;
+            while (x < key[j]) j--;
+            if (i <= j) {
+                int tk = key[i]; key[i] = key[j]; key[j] = tk;
+                int tv = val[i]; val[i] = val[j]; val[j] = tv;
+                i++;
+                j--;
+            }
+        } while (i <= j);
+        // recursion order of the reference: (L, j) first, then (i, H); a stack
+        // reverses, so push (i, H) first.
+        if (i < H && sp < 64) { stk_lo[sp] = i; stk_hi[sp] = H; sp++; }
+        if (L < j && sp < 64) { stk_lo[sp] = L; stk_hi[sp] = j; sp++; }
+    }
+}
+
+RSX_HD int uf_find(int* id, int x) {
+    while (x != id[x]) {
+        int p = id[id[x]];
+        id[x] = p;
+        x = p;
+    }
+    return x;
+}
+
+template <int MAXC>
+struct TickScratch {
+    MfList L;
+    Solver<MAXC> S;
+    V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
+    int cand[48];
+};
+
+template <int MAXC>
+RSX_HD Xf body_xf(const Arena<MAXC>& A, int code) {
+    if (code == 0) return A.ball.xf;
+    if (code >= 1 && code <= MAXC) return A.car[code - 1].xf;
+    Xf t = xf_identity();
+    int s = code - kBodyStatic;
+    if (s >= kNumMeshes) t.o = plane_origin(s - kNumMeshes);
+    return t;
+}
+
+// Contact-added callback (Arena::_BulletContactAddedCallback, Arena.cpp:218-427)
+template <int MAXC>
+RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx) {
+    Contact& cp = m.pt[idx];
+    int b0 = m.body0, b1 = m.body1;
+    bool st1 = b1 >= kBodyStatic;
+    if (b0 == 0 && st1) {
+        // ball + world: special resolution (btAdjustInternalEdgeContacts not yet applied)
+        cp.special = true;
+        return;
+    }
+    if (b0 >= 1 && b0 <= MAXC && st1) {
+        Car& c = A.car[b0 - 1];
+        c.world_contact = true;
+        c.world_contact_normal = cp.normal;
+        cp.friction = kCarWorldFriction;
+        cp.restitution = kCarWorldRestitution;
+        return;
+    }
+    if (b0 == 0 && b1 >= 1 && b1 <= MAXC) {
+        // car + ball (Arena.cpp:283-334); the ball is manifold body A
+        Car& c = A.car[b1 - 1];
+        Ball& ball = A.ball;
+        cp.friction = kCarBallFriction;
+        cp.restitution = kCarBallRestitution;
+        c.ball_hit_valid = true;
+        c.ball_hit_rel_pos = cp.localA * kBT2UU;
+        c.ball_hit_tick = A.tick;
+        V3 ball_pos = ball.xf.o * kBT2UU;
+        c.ball_hit_ball_pos = ball_pos;
+        c.ball_hit_extra_vel = V3{0, 0, 0};
+        if ((A.tick > c.ball_hit_extra_tick + 1) || (c.ball_hit_extra_tick > A.tick)) {
+            c.ball_hit_extra_tick = A.tick;
+        } else {
+            return;
+        }
+        V3 car_fwd_v = col(c.xf.b, 0);
+        V3 rel_pos = ball_pos - c.xf.o * kBT2UU;
+        V3 rel_vel = ball.vel * kBT2UU - c.vel * kBT2UU;
+        float rel_speed = minf(len(rel_vel), kBallCarExtraImpulseMaxDeltaVel);
+        if (rel_speed > 0) {
+            V3 hit_dir = safe_normalized(rel_pos * V3{1, 1, kBallCarExtraImpulseZScale});
+            V3 adj = car_fwd_v * dot(hit_dir, car_fwd_v) * (1 - kBallCarExtraImpulseForwardScale);
+            hit_dir = safe_normalized(hit_dir - adj);
+            V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_car_extra(), rel_speed) * 1.f;
+            c.ball_hit_extra_vel = added;
+            ball.vel_cache += added * kUU2BT;
+        }
+        return;
+    }
+    (void)T;
+}
+
+// Narrowphase of the ball against mesh `mesh` (convex-concave, swapped).
+template <int MAXC>
+RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh, const int* cand, int ncand) {
+    const Ball& b = A.ball;
+    Xf trB = xf_identity();
+    for (int i = 0; i < ncand; i++) {
+        int t = cand[i];
+        if (T.mv.tri_mesh[t] != mesh) continue;
+        V3 v0, v1, v2;
+        load_tri(T.mv, t, v0, v1, v2);
+        if (sphere_tri_early_out(b.xf.o, T.bg.radius, v0, v1, v2, m.breaking)) continue;
+        V3 pt, nrm;
+        float depth;
+        if (sphere_triangle(b.xf.o, T.bg.radius, m.breaking, v0, v1, v2, pt, nrm, depth)) {
+            int idx = manifold_add(m, b.xf, trB, nrm, pt, depth, minf(kBallFriction, 0.6f), maxf(kBallRestitution, 0.3f));
+            if (idx >= 0) contact_added(A, T, m, idx);
+        }
+    }
+    manifold_refresh(m, b.xf, trB);
+}
+
+// GJK result for the ball's centre point against the car's core box (margins
+// added back as btGjkPairDetector does).
+RSX_HD bool sphere_box(V3 c, float r, const Xf& box, V3 h, float margin, float breaking, V3& normal, V3& point_b,
+                       float& dist) {
+    V3 p = inv_xform(box, c);
+    V3 q = V3{clampf(p.x, -h.x, h.x), clampf(p.y, -h.y, h.y), clampf(p.z, -h.z, h.z)};
+    bool inside = (q == p);
+    if (!inside) {
+        V3 qw = xform(box, q);
+        V3 axis = c - qw;
+        float s2 = len2(axis);
+        float s = sqrtf(s2);
+        float marg = r + margin;
+        // early exit of GJK when clearly separated
+        float maxd = r + margin + breaking;
+        if (s2 > maxd * maxd) return false;
+        normal = axis * (1.f / s);
+        point_b = qw + axis * (margin / s);
+        dist = s - marg;
+        return true;
+    }
+    // centre inside the core box: push out along the nearest face
+    float dx = h.x - fabsf(p.x), dy = h.y - fabsf(p.y), dz = h.z - fabsf(p.z);
+    V3 ln;
+    float d;
+    V3 face = p;
+    if (dx <= dy && dx <= dz) { ln = V3{p.x >= 0 ? 1.f : -1.f, 0, 0}; d = dx; face.x = ln.x * h.x; }
+    else if (dy <= dz) { ln = V3{0, p.y >= 0 ? 1.f : -1.f, 0}; d = dy; face.y = ln.y * h.y; }
+    else { ln = V3{0, 0, p.z >= 0 ? 1.f : -1.f}; d = dz; face.z = ln.z * h.z; }
+    normal = box.b * ln;
+    point_b = xform(box, face) + normal * margin;
+    dist = -d - r - margin;
+    return true;
+}
+
+template <int MAXC>
+RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
+    const float dt = T.dt;
+    const int nb = 1 + A.ncars;
+    const CarGeom& g = T.cg;
+    // gravity on active bodies (applyGravity): m_gravity = acc * (1 / invMass)
+    const V3 gacc = V3{0, 0, kGravityZ} * kUU2BT;
+    if (!A.ball.sleeping) A.ball.force += gacc * (1.0f / T.bg.inv_mass);
+    for (int i = 0; i < A.ncars; i++)
+        if (A.car[i].active) A.car[i].force += gacc * (1.0f / g.inv_mass);
+
+    // predictUnconstraintMotion: damping + predicted transforms
+    {
+        Ball& b = A.ball;
+        b.vel *= powf(1.f - kBallDrag, dt);
+        b.ang *= powf(1.f - 0.f, dt);
+        b.interp = Xf{b.xf.b, b.xf.o + b.vel * dt};
+    }
+    for (int i = 0; i < A.ncars; i++) {
+        Car& c = A.car[i];
+        c.vel *= powf(1.f - 0.f, dt);
+        c.ang *= powf(1.f - 0.f, dt);
+        c.interp = integrate_transform(c.xf, c.vel, c.ang, dt);
+    }
+
+    // broadphase AABBs (updateSingleAabb, continuous union)
+    {
+        V3 a0, a1, b0, b1;
+        ball_shape_aabb(T.bg, A.ball.xf.o, a0, a1);
+        ball_shape_aabb(T.bg, A.ball.interp.o, b0, b1);
+        broad_aabb(a0, a1, b0, b1, W.bmin[0], W.bmax[0]);
+        for (int i = 0; i < A.ncars; i++) {
+            car_shape_aabb(g, A.car[i].xf, a0, a1);
+            car_shape_aabb(g, A.car[i].interp, b0, b1);
+            broad_aabb(a0, a1, b0, b1, W.bmin[1 + i], W.bmax[1 + i]);
+        }
+    }
+
+    // ---- pairs, manifolds, contacts (dispatch order) ----
+    MfList& L = W.L;
+    L.count = 0;
+    L.nlive = 0;
+    int uf[MAXC + 1];
+    for (int i = 0; i < nb; i++) uf[i] = i;
+    bool active[MAXC + 1];
+    active[0] = !A.ball.sleeping;
+    for (int i = 0; i < A.ncars; i++) active[1 + i] = A.car[i].active;
+
+    for (int d = 0; d < nb; d++) {
+        bool demoed = (d >= 1) && A.car[d - 1].is_demoed;
+        if (demoed) continue;  // no contact response: never solved
+        const int cell = bp_cell_index(T.mv, W.bmin[d]);
+        const uint16_t mask = T.mv.bp_meshes[cell];
+        // statics: meshes then planes
+        int ncand = -1;
+        for (int ms = 0; ms < kNumMeshes; ms++) {
+            if (!((mask >> ms) & 1)) continue;
+            const float* ma = T.mv.mesh_aabb + ms * 6;
+            const float th = kContactBreakingThreshold;
+            V3 smn = V3{ma[0], ma[1], ma[2]}, smx = V3{ma[3], ma[4], ma[5]};  // shape AABB
+            if (!aabb_overlap(W.bmin[d], W.bmax[d], smn - V3{th, th, th}, smx + V3{th, th, th})) continue;
+            if (!active[d]) continue;  // needsCollision: both inactive
+            float breaking = (d == 0) ? T.bg.breaking : g.breaking;
+            if (d >= 1) {
+                // compound child AABB against the mesh's shape AABB (no threshold)
+                Xf child = car_child_xf(g, A.car[d - 1].xf);
+                V3 cmn, cmx;
+                box_child_aabb(g, child, cmn, cmx);
+                if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
+            }
+            int k = mf_open(L, d, -1, breaking);
+            int s = L.slot[k];
+            if (s >= 0) {
+                Manifold& m = L.live[s];
+                m.body1 = kBodyStatic + ms;
+                if (d == 0) {
+                    if (ncand < 0) {
+                        V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
+                        ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
+                    }
+                    ball_vs_mesh(A, T, m, ms, W.cand, ncand);
+                }
+                // car vs mesh triangles: not generated yet
+            }
+            mf_close(L, k);
+        }
+        for (int p = 0; p < kNumPlanes; p++) {
+            V3 pmn, pmx;
+            plane_shape_aabb(p, pmn, pmx);
+            {
+                const float th = kContactBreakingThreshold;
+                if (!aabb_overlap(W.bmin[d], W.bmax[d], pmn - V3{th, th, th}, pmx + V3{th, th, th})) continue;
+            }
+            if (!active[d]) continue;
+            float breaking = (d == 0) ? T.bg.breaking : g.breaking;
+            if (d >= 1) {
+                // compound child AABB test against the plane's own AABB (no threshold)
+                Xf child = car_child_xf(g, A.car[d - 1].xf);
+                V3 cmn, cmx;
+                box_child_aabb(g, child, cmn, cmx);
+                // a child whose AABB misses is dropped with its manifold
+                if (!aabb_overlap(cmn, cmx, pmn, pmx)) continue;
+            }
+            int k = mf_open(L, d, -1, breaking);
+            int s = L.slot[k];
+            if (s >= 0) {
+                Manifold& m = L.live[s];
+                m.body1 = kBodyStatic + kNumMeshes + p;
+                Xf trB = Xf{m3_identity(), plane_origin(p)};
+                if (d == 0) {
+                    V3 nrm, pt;
+                    float dist;
+                    if (sphere_plane(T.bg, A.ball.xf.o, p, m.breaking, nrm, pt, dist)) {
+                        int idx = manifold_add(m, A.ball.xf, trB, nrm, pt, dist, minf(kBallFriction, 0.6f),
+                                               maxf(kBallRestitution, 0.3f));
+                        if (idx >= 0) contact_added(A, T, m, idx);
+                    }
+                    if (m.n) manifold_refresh(m, A.ball.xf, trB);
+                } else {
+                    Car& c = A.car[d - 1];
+                    Xf child = car_child_xf(g, c.xf);
+                    V3 pn = plane_normal(p);
+                    Xf plane_in_convex = xf_mul(xf_inverse(child), trB);
+                    Xf convex_in_plane = xf_mul(xf_inverse(trB), child);
+                    V3 dir = plane_in_convex.b * (-pn);
+                    V3 he = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+                    V3 vtx = V3{dir.x >= 0 ? he.x : -he.x, dir.y >= 0 ? he.y : -he.y, dir.z >= 0 ? he.z : -he.z};
+                    V3 vip = xform(convex_in_plane, vtx);
+                    float dist = dot(pn, vip) - 0.f;
+                    if (dist < m.breaking) {
+                        V3 proj = vip - dist * pn;
+                        V3 pt = xform(trB, proj);
+                        V3 nrm = trB.b * pn;
+                        int idx = manifold_add(m, c.xf, trB, nrm, pt, dist, 0.6f, 0.3f);
+                        if (idx >= 0) contact_added(A, T, m, idx);
+                    }
+                    if (m.n) manifold_refresh(m, c.xf, trB);
+                }
+            }
+            mf_close(L, k);
+        }
+        // dynamic pairs with later bodies
+        for (int e = d + 1; e < nb; e++) {
+            bool edemo = (e >= 1) && A.car[e - 1].is_demoed;
+            if (edemo) continue;
+            if (!aabb_overlap(W.bmin[d], W.bmax[d], W.bmin[e], W.bmax[e])) continue;
+            // islands: unite(d, e)
+            {
+                int i = uf_find(uf, d), j = uf_find(uf, e);
+                if (i != j) uf[i] = j;
+            }
+            if (!active[d] && !active[e]) continue;
+            if (d == 0) {
+                // ball vs car e-1: compound child AABB test against the ball's shape AABB
+                Car& c = A.car[e - 1];
+                Xf child = car_child_xf(g, c.xf);
+                V3 cmn, cmx, smn, smx;
+                box_child_aabb(g, child, cmn, cmx);
+                ball_shape_aabb(T.bg, A.ball.xf.o, smn, smx);
+                if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
+                float breaking = minf(T.bg.breaking, g.breaking);
+                int k = mf_open(L, 0, e, breaking);
+                int s = L.slot[k];
+                if (s >= 0) {
+                    Manifold& m = L.live[s];
+                    V3 nrm, ptb;
+                    float dist;
+                    if (sphere_box(A.ball.xf.o, T.bg.radius, child, g.box_implicit, g.box_margin, m.breaking, nrm, ptb,
+                                   dist)) {
+                        int idx = manifold_add(m, A.ball.xf, c.xf, nrm, ptb, dist, kBallFriction * kCarCollisionFriction,
+                                               kBallRestitution * kCarCollisionRestitution);
+                        if (idx >= 0) contact_added(A, T, m, idx);
+                    }
+                    if (m.n) manifold_refresh(m, A.ball.xf, c.xf);
+                }
+                mf_close(L, k);
+            }
+            // car vs car contacts: not generated yet
+        }
+    }
+
+    // ---- islands and manifold order (btSimulationIslandManager) ----
+    int tag[MAXC + 1];
+    for (int i = 0; i < nb; i++) tag[i] = uf_find(uf, i);
+    // a sleeping ball joined to an awake island wakes for this tick
+    bool island_awake[MAXC + 1];
+    for (int i = 0; i < nb; i++) island_awake[i] = false;
+    for (int i = 0; i < nb; i++)
+        if (active[i]) island_awake[tag[i]] = true;
+    bool in_solver[MAXC + 1];
+    for (int i = 0; i < nb; i++) in_solver[i] = island_awake[tag[i]] && !((i >= 1) && A.car[i - 1].is_demoed);
+    bool ball_woken = A.ball.sleeping && in_solver[0];
+
+    int keys[kMaxManifolds], order[kMaxManifolds];
+    int nm = 0;
+    for (int k = 0; k < L.count; k++) {
+        int b0 = L.body0[k];
+        keys[nm] = tag[b0];
+        order[nm] = k;
+        nm++;
+    }
+    if (nm > 1) bt_quicksort(keys, order, 0, nm - 1);
+
+    // ---- solver setup ----
+    Solver<MAXC>& S = W.S;
+    S.nrow = 0;
+    S.nfric = 0;
+    const int FIX = MAXC + 1;
+    for (int i = 0; i < MAXC + 2; i++) S.body[i].used = false;
+    sbody_init_fixed(S.body[FIX]);
+    if (in_solver[0]) {
+        Ball& b = A.ball;
+        M3 ii = m3(T.bg.inv_inertia, 0, 0, 0, T.bg.inv_inertia, 0, 0, 0, T.bg.inv_inertia);
+        sbody_init(S.body[0], b.xf, b.vel, b.ang, b.force, V3{0, 0, 0}, T.bg.inv_mass, ii, dt);
+    }
+    for (int i = 0; i < A.ncars; i++) {
+        if (!in_solver[1 + i]) continue;
+        Car& c = A.car[i];
+        sbody_init(S.body[1 + i], c.xf, c.vel, c.ang, c.force, c.torque, g.inv_mass, c.inv_inertia_world, dt);
+    }
+    for (int i = 0; i <= MAXC; i++) {
+        S.nspecial[i] = 0;
+        S.sp_normal[i] = V3{0, 0, 0};
+        S.sp_dist[i] = 0;
+        S.sp_friction[i] = 0;
+        S.sp_restitution[i] = 0;
+    }
+    for (int oi = 0; oi < nm; oi++) {
+        int k = order[oi];
+        int s = L.slot[k];
+        if (s < 0) continue;
+        const Manifold& m = L.live[s];
+        int ia = m.body0;
+        int ib = (m.body1 >= kBodyStatic) ? FIX : m.body1;
+        if (!in_solver[m.body0] && !(m.body1 < kBodyStatic && in_solver[m.body1])) continue;
+        Xf xa = body_xf(A, m.body0);
+        Xf xb = body_xf(A, m.body1);
+        for (int j = 0; j < m.n; j++) {
+            if (S.nrow >= kMaxRows) break;
+            const Contact& cp = m.pt[j];
+            V3 rel1 = cp.posA - xa.o;
+            V3 rel2 = cp.posB - xb.o;
+            int ridx = S.nrow++;
+            Row& r = S.row[ridx];
+            setup_contact_row(S, r, ia, ib, cp.normal, cp.dist, cp.friction, cp.restitution, cp.special, rel1, rel2, dt);
+            r.fidx = (int16_t)S.nfric;
+            if (cp.special) {
+                // only non-static bodies accumulate (the world side is static here)
+                int bi = m.body0;
+                S.nspecial[bi]++;
+                S.sp_friction[bi] = cp.friction;
+                S.sp_restitution[bi] = cp.restitution;
+                S.sp_normal[bi] += cp.normal;
+                S.sp_dist[bi] += len(rel1);
+            }
+            add_contact_friction(S, ia, ib, cp.normal, cp.friction, rel1, rel2, ridx);
+        }
+    }
+    // special rows, in body order (only the ball produces them)
+    for (int bi = 0; bi < nb; bi++) {
+        if (S.nspecial[bi] <= 0 || S.nrow >= kMaxRows) continue;
+        float distance = S.sp_dist[bi] / (float)S.nspecial[bi];
+        V3 normal = S.sp_normal[bi] / (float)S.nspecial[bi];
+        int ridx = S.nrow++;
+        Row& r = S.row[ridx];
+        V3 rel1 = normal * -distance;
+        V3 rel2 = V3{0, 0, 0};
+        setup_contact_row(S, r, bi, FIX, normal, distance, S.sp_friction[bi], S.sp_restitution[bi], false, rel1, rel2,
+                          dt);
+        add_contact_friction(S, bi, FIX, normal, S.sp_friction[bi], rel1, rel2, ridx);
+    }
+
+    solver_iterate(S);
+
+    // write-back
+    if (in_solver[0]) {
+        Ball& b = A.ball;
+        sbody_writeback(S.body[0], true, dt, b.xf, b.vel, b.ang);
+    }
+    for (int i = 0; i < A.ncars; i++) {
+        if (!in_solver[1 + i]) continue;
+        Car& c = A.car[i];
+        sbody_writeback(S.body[1 + i], false, dt, c.xf, c.vel, c.ang);
+    }
+
+    // integrateTransforms (active bodies)
+    if (!A.ball.sleeping || ball_woken) {
+        Ball& b = A.ball;
+        b.xf.o = b.xf.o + b.vel * dt;
+    }
+    for (int i = 0; i < A.ncars; i++) {
+        Car& c = A.car[i];
+        if (!c.active) continue;
+        c.xf = integrate_transform(c.xf, c.vel, c.ang, dt);
+        car_update_inertia(c, g);
+    }
+    // clearForces
+    A.ball.force = V3{0, 0, 0};
+    for (int i = 0; i < A.ncars; i++) {
+        A.car[i].force = V3{0, 0, 0};
+        A.car[i].torque = V3{0, 0, 0};
+    }
+}
+
+// ---- boost pads (BoostPad.cpp:51-105, BoostPadGrid.cpp:5-25) -----------------
+template <int MAXC>
+RSX_HD void pad_check_car(Arena<MAXC>& A, const TickCtx& T, int ci) {
+    Car& c = A.car[ci];
+    if (c.is_demoed || c.boost >= 100) return;
+    V3 pos = c.xf.o * kBT2UU;
+    if (pos.z > kPadGridExtentZ) return;
+    int ix = (int)(pos.x / kPadCellSizeX + (kPadCellsX / 2));
+    int iy = (int)(pos.y / kPadCellSizeY + (kPadCellsY / 2));
+    V3 cmin, cmax;
+    car_shape_aabb(T.cg, c.xf, cmin, cmax);
+    for (int p = 0; p < kNumPads; p++) {
+        const PadGeom& pg = T.pads[p];
+        int px = (int)(pg.pos_uu.x / kPadCellSizeX + (kPadCellsX / 2));
+        int py = (int)(pg.pos_uu.y / kPadCellSizeY + (kPadCellsY / 2));
+        if (px < (ix - 1 > 0 ? ix - 1 : 0) || px > (ix + 1 < kPadCellsX - 1 ? ix + 1 : kPadCellsX - 1)) continue;
+        if (py < (iy - 1 > 0 ? iy - 1 : 0) || py > (iy + 1 < kPadCellsY - 1 ? iy + 1 : kPadCellsY - 1)) continue;
+        Pad& pad = A.pad[p];
+        bool colliding = false;
+        V3 cp = c.xf.o;
+        if (pad.prev_locked == c.id) {
+            colliding = (pg.box_max.x > cmin.x && pg.box_max.y > cmin.y && pg.box_max.z > cmin.z) &&
+                        (pg.box_min.x < cmax.x && pg.box_min.y < cmax.y && pg.box_min.z < cmax.z);
+        } else {
+            float dx = cp.x - pg.pos_bt.x, dy = cp.y - pg.pos_bt.y;
+            float r = pg.cyl_rad_bt;
+            if (dx * dx + dy * dy < (r * r)) colliding = fabsf(cp.z - pg.pos_bt.z) < (kPadCylHeight * kUU2BT);
+        }
+        if (colliding) pad.cur_locked = c.id;
+    }
+}
+
+template <int MAXC>
+RSX_HD Car* car_by_id(Arena<MAXC>& A, uint32_t id) {
+    for (int i = 0; i < A.ncars; i++)
+        if (A.car[i].id == id) return &A.car[i];
+    return nullptr;
+}
+
+RSX_HD uint32_t rng_next(uint32_t& s) {
+    // xorshift32
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    return s;
+}
+
+// Car::Respawn with the engine's own per-arena draw of the respawn slot.
+template <int MAXC>
+RSX_HD void car_respawn(Arena<MAXC>& A, const TickCtx& T, int ci);
+
+// One full tick (Arena::Step body).  Returns true if the ball is in a goal
+// after the tick (Arena::IsBallScored).
+template <int MAXC>
+RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
+    const float dt = T.dt;
+    // ball zero-velocity sleep
+    A.ball.sleeping = (len2(A.ball.vel) == 0 && len2(A.ball.ang) == 0);
+    for (int i = 0; i < A.ncars; i++) {
+        Car& c = A.car[i];
+        Controls& k = c.controls;
+        k.throttle = clampf(k.throttle, -1, 1);
+        k.steer = clampf(k.steer, -1, 1);
+        k.pitch = clampf(k.pitch, -1, 1);
+        k.yaw = clampf(k.yaw, -1, 1);
+        k.roll = clampf(k.roll, -1, 1);
+        c.active = true;
+        if (c.is_demoed) {
+            c.demo_respawn_timer = maxf(c.demo_respawn_timer - dt, 0);
+            if (c.demo_respawn_timer == 0) car_respawn(A, T, i);
+            c.active = false;  // body disabled for this tick
+        }
+        if (c.is_demoed) continue;
+        car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt);
+    }
+    if (A.ncars > 0) {
+        for (int p = 0; p < kNumPads; p++) {
+            Pad& pad = A.pad[p];
+            if (pad.cooldown > 0) pad.cooldown = maxf(pad.cooldown - dt, 0);
+            pad.is_active = (pad.cooldown == 0);
+            pad.cur_locked = 0;
+        }
+    }
+    arena_world_step(A, T, W);
+    for (int i = 0; i < A.ncars; i++) {
+        Car& c = A.car[i];
+        if (c.is_demoed) continue;
+        car_post_tick(c, dt);
+        car_finish_tick(c);
+        pad_check_car(A, T, i);
+    }
+    if (A.ncars > 0) {
+        for (int p = 0; p < kNumPads; p++) {
+            Pad& pad = A.pad[p];
+            uint32_t locked = 0;
+            if (pad.cur_locked) {
+                locked = pad.cur_locked;
+                if (pad.is_active) {
+                    Car* c = car_by_id(A, pad.cur_locked);
+                    float add = T.pads[p].big ? kPadAmountBig : kPadAmountSmall;
+                    if (c) c->boost = minf(c->boost + add, kBoostMax);
+                    pad.is_active = false;
+                    pad.cooldown = T.pads[p].big ? kPadCooldownBig : kPadCooldownSmall;
+                }
+            }
+            pad.prev_locked = locked;
+        }
+    }
+    {
+        Ball& b = A.ball;
+        if (!is_zero(b.vel_cache)) {
+            b.vel += b.vel_cache;
+            b.vel_cache = V3{0, 0, 0};
+        }
+        V3 v = b.vel, a = b.ang;
+        float ms = kBallMaxSpeed * kUU2BT;
+        if (len2(v) > ms * ms) v = normalized(v) * ms;
+        if (len2(a) > (kBallMaxAngSpeed * kBallMaxAngSpeed)) a = normalized(a) * kBallMaxAngSpeed;
+        b.vel = v;
+        b.ang = a;
+    }
+    bool scored = fabsf(A.ball.xf.o.y * kBT2UU) > (kGoalThresholdY + kBallRadius);
+    A.tick++;
+    return scored;
+}
+
+// Respawn spots (RLConst.h CAR_RESPAWN_LOCATIONS_SOCCAR), blue side.
+RSX_HD void respawn_spot(int idx, float& x, float& y, float& yaw) {
+    const float xs[4] = {-2304, -2688, 2304, 2688};
+    x = xs[idx & 3];
+    y = -4608;
+    yaw = (float)(3.14159265358979323846 / 2);
+}
+
+RSX_HD M3 yaw_basis(float yaw) {
+    // Angle(yaw, 0, 0).ToRotMat(): btMatrix3x3::setEulerYPR(yaw, -0, -0)
+    return m3_euler_zyx(-0.f, -0.f, yaw);
+}
+
+template <int MAXC>
+RSX_HD void car_set_spawn(Car& c, const CarGeom& g, float x, float y, float z, float yaw, float boost) {
+    V3 zero = V3{0, 0, 0};
+    c.xf.o = V3{x, y, z} * kUU2BT;
+    c.xf.b = yaw_basis(yaw);
+    c.vel = c.ang = zero;
+    car_update_inertia(c, g);
+    c.vel_cache = zero;
+    c.force = c.torque = zero;
+    c.on_ground = true;
+    c.has_jumped = c.has_double_jumped = c.has_flipped = c.is_flipping = c.is_jumping = false;
+    c.is_supersonic = c.is_auto_flipping = c.is_demoed = c.world_contact = c.ball_hit_valid = false;
+    for (int i = 0; i < 4; i++) c.wheel_contact[i] = false;
+    c.boost = boost;
+    c.time_spent_boosting = c.jump_time = c.flip_time = c.air_time = c.air_time_since_jump = 0;
+    c.supersonic_time = c.handbrake_val = c.auto_flip_timer = c.auto_flip_torque_scale = 0;
+    c.demo_respawn_timer = c.car_contact_cooldown = 0;
+    c.flip_rel_torque = c.world_contact_normal = zero;
+    c.car_contact_other_id = 0;
+    c.ball_hit_tick = ~0ULL;
+    c.ball_hit_extra_tick = ~0ULL;
+    c.ball_hit_rel_pos = c.ball_hit_ball_pos = c.ball_hit_extra_vel = zero;
+    c.last_controls = Controls{0, 0, 0, 0, 0, false, false, false};
+}
+
+template <int MAXC>
+RSX_HD void car_respawn(Arena<MAXC>& A, const TickCtx& T, int ci) {
+    Car& c = A.car[ci];
+    int idx = (int)(rng_next(A.rng) % 4u);
+    float x, y, yaw;
+    respawn_spot(idx, x, y, yaw);
+    float sgn = c.team == 0 ? 1.f : -1.f;
+    float yw = yaw + (c.team == 0 ? 0.f : 3.14159265358979323846f);
+    car_set_spawn<MAXC>(c, T.cg, x, y * sgn, kCarRespawnZ, yw, kBoostSpawnAmount);
+}
+
+}  // namespace rsx

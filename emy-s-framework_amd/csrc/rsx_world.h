@@ -54,7 +54,7 @@ struct MeshView {
     const int* cell_tris;      // triangle ids
     const uint8_t* susp_world; // suspension grid cells with world geometry (bled), 32*56*8
     const uint16_t* bp_meshes; // broadphase grid: per cell, mask of meshes listed in the cell
-    const float* mesh_aabb;    // [16][6] min xyz, max xyz (BT)
+    const float* mesh_aabb;    // [16][6] shape AABB min xyz, max xyz (BT)
     V3 grid_min;               // uniform grid origin (BT)
     float grid_cell;           // cell size (BT)
     int gx, gy, gz;
@@ -143,6 +143,18 @@ struct Manifold {
     int n;
     float breaking;
     Contact pt[kMaxManifoldPoints];
+};
+
+// Full working state of one arena.  MAXC is the car capacity of the arena
+// (2, 4 or 6 for 1v1, 2v2, 3v3); ncars is the number in use.
+template <int MAXC>
+struct Arena {
+    int ncars;
+    uint64_t tick;
+    Car car[MAXC];
+    Ball ball;
+    Pad pad[kNumPads];
+    uint32_t rng;              // per-arena RNG state (kickoff / demo respawn draws)
 };
 
 }  // namespace rsx

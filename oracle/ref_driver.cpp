@@ -294,6 +294,7 @@ void ref_set_wheels(void* p, int i, const float* in) {
         wi.m_brake = o[2];
         wi.m_latFriction = o[3];
         wi.m_longFriction = o[4];
+        wi.m_extraPushback = o[8];
     }
 }
 
@@ -366,3 +367,17 @@ void ref_get_aabb(void* p, int i, float* out6) {
 }
 
 }  // extern "C"
+
+// World ray test (btCollisionWorld::rayTest with a closest-hit callback) from
+// a to b in Bullet units; out: hit(1/0), fraction, point(3), normal(3), body code.
+extern "C" void ref_raytest(void* p, const float* a, const float* b, float* out) {
+    auto* h = (RefArena*)p;
+    btVector3 from(a[0], a[1], a[2]), to(b[0], b[1], b[2]);
+    btCollisionWorld::ClosestRayResultCallback cb(from, to, nullptr);
+    h->arena->_bulletWorld.rayTest(from, to, cb);
+    out[0] = cb.hasHit() ? 1.f : 0.f;
+    out[1] = cb.m_closestHitFraction;
+    out[2] = cb.m_hitPointWorld.x(); out[3] = cb.m_hitPointWorld.y(); out[4] = cb.m_hitPointWorld.z();
+    out[5] = cb.m_hitNormalWorld.x(); out[6] = cb.m_hitNormalWorld.y(); out[7] = cb.m_hitNormalWorld.z();
+    out[8] = cb.hasHit() ? (float)body_code(h, cb.m_collisionObject) : -1.f;
+}

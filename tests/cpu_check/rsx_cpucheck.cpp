@@ -1,0 +1,182 @@
+// rsx_cpucheck.cpp — TEST INFRASTRUCTURE: a host (CPU) build of the engine's
+// per-arena tick code (the same headers the gfx950 kernels compile), exposed as
+// plain C so the CPU-side tests can compare it with the oracle without a GPU.
+// Nothing in the engine's product path loads this library.
+#include "../../emy-s-framework_amd/csrc/rsx_host.h"
+
+#include <cstring>
+#include <string>
+
+using namespace rsx;
+
+struct CcWorld {
+    HostWorld w;
+    MeshView mv;
+};
+
+struct CcArena {
+    CcWorld* world;
+    Arena<6> A;
+    TickScratch<6> W;
+    int last_scored;
+};
+
+extern "C" {
+
+void* cc_world_new(const char* dir) {
+    auto* w = new CcWorld();
+    std::string err;
+    if (!load_world(dir, w->w, err)) {
+        delete w;
+        return nullptr;
+    }
+    w->mv = w->w.view();
+    return w;
+}
+
+void cc_world_free(void* p) { delete (CcWorld*)p; }
+
+void cc_mesh_hashes(void* p, uint32_t* out16) {
+    auto* w = (CcWorld*)p;
+    for (int i = 0; i < 16; i++) out16[i] = w->w.mesh_hash[i];
+}
+
+void* cc_arena_new(void* world, int n_blue, int n_orange) {
+    auto* a = new CcArena();
+    memset(&a->A, 0, sizeof(a->A));
+    a->world = (CcWorld*)world;
+    a->A.ncars = n_blue + n_orange;
+    a->A.rng = 0x9E3779B9u;
+    for (int i = 0; i < a->A.ncars; i++) {
+        Car& c = a->A.car[i];
+        c.team = i < n_blue ? 0 : 1;
+        c.id = (uint32_t)(i + 1);
+        car_set_spawn<6>(c, a->world->w.cg, 0, 0, kCarSpawnRestZ, 0, kBoostSpawnAmount);
+    }
+    a->A.ball.xf = xf_identity();
+    a->A.ball.xf.o = V3{0, 0, kBallRestZ * kUU2BT};
+    for (int p = 0; p < kNumPads; p++) a->A.pad[p] = Pad{true, 0.f, 0, 0};
+    return a;
+}
+
+void cc_arena_free(void* p) { delete (CcArena*)p; }
+
+void cc_set_car(void* p, int i, const rsx_car_state* s) {
+    auto* a = (CcArena*)p;
+    Car& c = a->A.car[i];
+    uint32_t id = c.id;
+    int team = c.team;
+    car_from_public(c, *s, a->world->w.cg);
+    c.id = id;
+    c.team = team;
+}
+
+void cc_get_car(void* p, int i, rsx_car_state* s) {
+    auto* a = (CcArena*)p;
+    car_to_public(a->A.car[i], *s);
+}
+
+void cc_set_ball(void* p, const rsx_ball_state* s) { ball_from_public(((CcArena*)p)->A.ball, *s); }
+void cc_get_ball(void* p, rsx_ball_state* s) { ball_to_public(((CcArena*)p)->A.ball, *s); }
+
+void cc_set_pads(void* p, const rsx_pad_state* s) {
+    auto* a = (CcArena*)p;
+    for (int i = 0; i < kNumPads; i++) a->A.pad[i] = Pad{s[i].is_active != 0, s[i].cooldown, s[i].prev_locked_car_id, 0};
+}
+void cc_get_pads(void* p, rsx_pad_state* s) {
+    auto* a = (CcArena*)p;
+    for (int i = 0; i < kNumPads; i++) {
+        memset(&s[i], 0, sizeof(s[i]));
+        s[i].is_active = a->A.pad[i].is_active;
+        s[i].cooldown = a->A.pad[i].cooldown;
+        s[i].prev_locked_car_id = a->A.pad[i].prev_locked;
+    }
+}
+
+void cc_set_controls(void* p, int i, const rsx_controls* c) {
+    ((CcArena*)p)->A.car[i].controls = controls_from_public(*c);
+}
+
+void cc_set_tick(void* p, uint64_t t) { ((CcArena*)p)->A.tick = t; }
+uint64_t cc_get_tick(void* p) { return ((CcArena*)p)->A.tick; }
+
+// Per-wheel persistent values: steer, engine, brake, lat, long, extra pushback.
+void cc_set_wheels(void* p, int i, const float* in) {
+    auto* a = (CcArena*)p;
+    for (int w = 0; w < 4; w++) {
+        Wheel& wh = a->A.car[i].wheel[w];
+        wh.steer_angle = in[w * 6 + 0];
+        wh.engine_force = in[w * 6 + 1];
+        wh.brake = in[w * 6 + 2];
+        wh.lat_friction = in[w * 6 + 3];
+        wh.long_friction = in[w * 6 + 4];
+        wh.extra_pushback = in[w * 6 + 5];
+    }
+}
+void cc_get_wheels(void* p, int i, float* out) {
+    auto* a = (CcArena*)p;
+    for (int w = 0; w < 4; w++) {
+        const Wheel& wh = a->A.car[i].wheel[w];
+        float* o = out + w * 12;
+        o[0] = wh.steer_angle; o[1] = wh.engine_force; o[2] = wh.brake; o[3] = wh.lat_friction;
+        o[4] = wh.long_friction; o[5] = wh.susp_len; o[6] = wh.susp_rel_vel; o[7] = wh.clipped_inv_dot;
+        o[8] = wh.extra_pushback; o[9] = wh.in_contact; o[10] = wh.susp_force; o[11] = wh.in_contact_world;
+    }
+}
+
+int cc_step(void* p, int ticks) {
+    auto* a = (CcArena*)p;
+    TickCtx T;
+    T.cg = a->world->w.cg;
+    T.bg = a->world->w.bg;
+    T.mv = a->world->mv;
+    T.pads = a->world->w.pads;
+    T.dt = 1.f / kTickRate;
+    int scored = 0;
+    for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
+    a->last_scored = scored;
+    return scored;
+}
+
+// Manifolds with contacts from the last tick, same record layout as the
+// oracle's ref_dump_contacts (body codes: static s -> s, ball 100, car i 200+i).
+int cc_dump_contacts(void* p, float* out, int max_floats) {
+    auto* a = (CcArena*)p;
+    const MfList& L = a->W.L;
+    int n = 0;
+    auto code = [](int b) { return b >= kBodyStatic ? b - kBodyStatic : (b == 0 ? 100 : 200 + b - 1); };
+    for (int k = 0; k < L.count; k++) {
+        int s = L.slot[k];
+        if (s < 0) continue;
+        const Manifold& m = L.live[s];
+        if (n + 4 + m.n * 22 > max_floats) break;
+        out[n++] = (float)code(m.body0);
+        out[n++] = (float)code(m.body1);
+        out[n++] = (float)m.n;
+        out[n++] = m.breaking;
+        for (int i = 0; i < m.n; i++) {
+            const Contact& c = m.pt[i];
+            const V3 vals[] = {c.posA, c.posB, c.normal};
+            for (auto& v : vals) { out[n++] = v.x; out[n++] = v.y; out[n++] = v.z; }
+            out[n++] = c.dist; out[n++] = c.friction; out[n++] = c.restitution; out[n++] = c.special;
+            out[n++] = 0.f;
+            const V3 loc[] = {c.localA, c.localB};
+            for (auto& v : loc) { out[n++] = v.x; out[n++] = v.y; out[n++] = v.z; }
+            out[n++] = 0.f; out[n++] = 0.f;
+        }
+    }
+    return n;
+}
+
+}  // extern "C"
+
+extern "C" void cc_raytest_static(void* w, const float* a, const float* b, float* out) {
+    auto* W = (CcWorld*)w;
+    RayHit hit;
+    hit.object = -1;
+    hit.fraction = 1.f;
+    ray_static_world(W->mv, V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}, hit);
+    out[0] = hit.object >= 0;
+    out[1] = hit.fraction;
+    out[5] = hit.normal.x; out[6] = hit.normal.y; out[7] = hit.normal.z;
+}

@@ -1,0 +1,161 @@
+"""ctypes bindings used by the tests: the oracle (reference CPU RocketSim built
+from source into oracle/_ref/librsx_ref.so), the CPU build of the engine's tick
+code (tests/cpu_check/librsx_cpucheck.so), and the record types of
+include/rsx_types.h."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+MESH_DIR = os.path.join(ROOT, "emy-s-framework_amd", "assets", "soccar")
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+    def np(self):
+        return np.array([self.x, self.y, self.z], dtype=np.float32)
+
+    def set(self, v):
+        self.x, self.y, self.z = (float(v[0]), float(v[1]), float(v[2]))
+
+
+class RotMat(C.Structure):
+    _fields_ = [("forward", Vec3), ("right", Vec3), ("up", Vec3)]
+
+
+class Controls(C.Structure):
+    _fields_ = [("throttle", C.c_float), ("steer", C.c_float), ("pitch", C.c_float), ("yaw", C.c_float),
+                ("roll", C.c_float), ("jump", C.c_uint8), ("boost", C.c_uint8), ("handbrake", C.c_uint8),
+                ("_pad", C.c_uint8)]
+
+
+class CarState(C.Structure):
+    _fields_ = [("pos", Vec3), ("rot", RotMat), ("vel", Vec3), ("ang_vel", Vec3),
+                ("boost", C.c_float), ("time_spent_boosting", C.c_float), ("jump_time", C.c_float),
+                ("flip_time", C.c_float), ("air_time", C.c_float), ("air_time_since_jump", C.c_float),
+                ("supersonic_time", C.c_float), ("handbrake_val", C.c_float), ("auto_flip_timer", C.c_float),
+                ("auto_flip_torque_scale", C.c_float), ("demo_respawn_timer", C.c_float),
+                ("car_contact_cooldown", C.c_float), ("flip_rel_torque", Vec3), ("world_contact_normal", Vec3),
+                ("is_on_ground", C.c_uint8), ("has_jumped", C.c_uint8), ("has_double_jumped", C.c_uint8),
+                ("has_flipped", C.c_uint8), ("is_flipping", C.c_uint8), ("is_jumping", C.c_uint8),
+                ("is_supersonic", C.c_uint8), ("is_auto_flipping", C.c_uint8), ("is_demoed", C.c_uint8),
+                ("world_contact", C.c_uint8), ("ball_hit_valid", C.c_uint8), ("team", C.c_uint8),
+                ("wheels_with_contact", C.c_uint8 * 4), ("id", C.c_uint32), ("car_contact_other_id", C.c_uint32),
+                ("ball_hit_tick", C.c_uint64), ("ball_hit_extra_impulse_tick", C.c_uint64),
+                ("ball_hit_rel_pos", Vec3), ("ball_hit_ball_pos", Vec3), ("ball_hit_extra_vel", Vec3),
+                ("last_controls", Controls)]
+
+
+class BallState(C.Structure):
+    _fields_ = [("pos", Vec3), ("vel", Vec3), ("ang_vel", Vec3)]
+
+
+class PadState(C.Structure):
+    _fields_ = [("is_active", C.c_uint8), ("_pad", C.c_uint8 * 3), ("cooldown", C.c_float),
+                ("prev_locked_car_id", C.c_uint32)]
+
+
+PADS = PadState * 34
+
+CAR_FLOAT_FIELDS = ["boost", "time_spent_boosting", "jump_time", "flip_time", "air_time", "air_time_since_jump",
+                    "supersonic_time", "handbrake_val", "auto_flip_timer", "auto_flip_torque_scale",
+                    "demo_respawn_timer", "car_contact_cooldown"]
+CAR_FLAG_FIELDS = ["is_on_ground", "has_jumped", "has_double_jumped", "has_flipped", "is_flipping", "is_jumping",
+                   "is_supersonic", "is_auto_flipping", "is_demoed", "world_contact", "ball_hit_valid"]
+
+
+def car_vec(s):
+    """pos(3) fwd(3) right(3) up(3) vel(3) angvel(3) as float64."""
+    return np.array([s.pos.x, s.pos.y, s.pos.z, s.rot.forward.x, s.rot.forward.y, s.rot.forward.z,
+                     s.rot.right.x, s.rot.right.y, s.rot.right.z, s.rot.up.x, s.rot.up.y, s.rot.up.z,
+                     s.vel.x, s.vel.y, s.vel.z, s.ang_vel.x, s.ang_vel.y, s.ang_vel.z], dtype=np.float64)
+
+
+def ball_vec(s):
+    return np.array([s.pos.x, s.pos.y, s.pos.z, s.vel.x, s.vel.y, s.vel.z, s.ang_vel.x, s.ang_vel.y,
+                     s.ang_vel.z], dtype=np.float64)
+
+
+def _lib(path, fns):
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    for name, res, args in fns:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+VP = C.c_void_p
+ORACLE_PATH = os.path.join(ROOT, "oracle", "_ref", "librsx_ref.so")
+CPUCHECK_PATH = os.path.join(ROOT, "tests", "cpu_check", "librsx_cpucheck.so")
+
+
+def load_oracle():
+    lib = _lib(ORACLE_PATH, [
+        ("ref_init", C.c_int, [C.c_char_p]),
+        ("ref_arena_new", VP, [C.c_int, C.c_int]),
+        ("ref_arena_free", None, [VP]),
+        ("ref_get_car", None, [VP, C.c_int, C.POINTER(CarState)]),
+        ("ref_set_car", None, [VP, C.c_int, C.POINTER(CarState)]),
+        ("ref_set_controls", None, [VP, C.c_int, C.POINTER(Controls)]),
+        ("ref_get_ball", None, [VP, C.POINTER(BallState)]),
+        ("ref_set_ball", None, [VP, C.POINTER(BallState)]),
+        ("ref_get_pads", None, [VP, C.POINTER(PadState)]),
+        ("ref_set_pads", None, [VP, C.POINTER(PadState)]),
+        ("ref_step", None, [VP, C.c_int]),
+        ("ref_get_tick", C.c_uint64, [VP]),
+        ("ref_set_tick", None, [VP, C.c_uint64]),
+        ("ref_ball_scored", C.c_int, [VP]),
+        ("ref_reset_kickoff", None, [VP, C.c_int]),
+        ("ref_get_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
+        ("ref_set_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
+        ("ref_num_contacts", C.c_int, [VP]),
+        ("ref_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+        ("ref_get_aabb", None, [VP, C.c_int, C.POINTER(C.c_float)]),
+        ("ref_pad_config", None, [VP, C.POINTER(C.c_float)]),
+        ("ref_sizeof_car_state", C.c_uint32, []),
+    ])
+    if lib is not None:
+        assert lib.ref_init(MESH_DIR.encode()) == 0
+    return lib
+
+
+def load_cpucheck():
+    return _lib(CPUCHECK_PATH, [
+        ("cc_world_new", VP, [C.c_char_p]),
+        ("cc_world_free", None, [VP]),
+        ("cc_mesh_hashes", None, [VP, C.POINTER(C.c_uint32)]),
+        ("cc_arena_new", VP, [VP, C.c_int, C.c_int]),
+        ("cc_arena_free", None, [VP]),
+        ("cc_set_car", None, [VP, C.c_int, C.POINTER(CarState)]),
+        ("cc_get_car", None, [VP, C.c_int, C.POINTER(CarState)]),
+        ("cc_set_ball", None, [VP, C.POINTER(BallState)]),
+        ("cc_get_ball", None, [VP, C.POINTER(BallState)]),
+        ("cc_set_pads", None, [VP, C.POINTER(PadState)]),
+        ("cc_get_pads", None, [VP, C.POINTER(PadState)]),
+        ("cc_set_controls", None, [VP, C.c_int, C.POINTER(Controls)]),
+        ("cc_set_tick", None, [VP, C.c_uint64]),
+        ("cc_get_tick", C.c_uint64, [VP]),
+        ("cc_set_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
+        ("cc_get_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
+        ("cc_step", C.c_int, [VP, C.c_int]),
+        ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+    ])
+
+
+def parse_contacts(buf, n):
+    out = []
+    i = 0
+    while i < n:
+        b0, b1, npt, brk = int(buf[i]), int(buf[i + 1]), int(buf[i + 2]), buf[i + 3]
+        i += 4
+        pts = []
+        for _ in range(npt):
+            pts.append(np.array(buf[i:i + 22], dtype=np.float64))
+            i += 22
+        out.append((b0, b1, brk, pts))
+    return out
