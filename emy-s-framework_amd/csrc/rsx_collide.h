@@ -26,6 +26,15 @@ namespace rsx {
 
 constexpr int kBodyStatic = 100;  // body code base of static bodies
 
+RSX_HD uint32_t __float_as_uint_rsx(float f) {
+    union {
+        float f;
+        uint32_t u;
+    } c;
+    c.f = f;
+    return c.u;
+}
+
 struct TickCtx {
     CarGeom cg;
     BallGeom bg;
@@ -213,6 +222,121 @@ RSX_HD int bp_cell_index(const MeshView& w, V3 pos) {
     j = j < 0 ? 0 : (j > w.bp_ny - 1 ? w.bp_ny - 1 : j);
     k = k < 0 ? 0 : (k > w.bp_nz - 1 ? w.bp_nz - 1 : k);
     return i * w.bp_ny * w.bp_nz + j * w.bp_nz + k;
+}
+
+
+// btAdjustInternalEdgeContacts (btInternalEdgeUtility.cpp:330-690) with
+// normalAdjustFlags = 0; the mesh transform is the identity.
+RSX_HD void nearest_on_segment(V3 p, V3 a, V3 b, V3& out) {
+    V3 d = b - a;
+    if (fuzzy_zero(d)) {
+        out = a;
+    } else {
+        float t = dot(p - a, d) / dot(d, d);
+        if (t < 0) t = 0;
+        else if (t > 1) t = 1;
+        out = a + d * t;
+    }
+}
+RSX_HD float get_angle(V3 edge_a, V3 normal_a, V3 normal_b) {
+    return atan2f(dot(normal_b, edge_a), dot(normal_b, normal_a));
+}
+RSX_HD V3 quat_rotate_v(Q4 r, V3 v) {
+    Q4 q = Q4{r.w * v.x + r.y * v.z - r.z * v.y, r.w * v.y + r.z * v.x - r.x * v.z, r.w * v.z + r.x * v.y - r.y * v.x,
+              -r.x * v.x - r.y * v.y - r.z * v.z};
+    q = qmul(q, qinverse(r));
+    return V3{q.x, q.y, q.z};
+}
+RSX_HD bool clamp_normal(V3 edge, V3 tri_normal, V3 n, float corrected, V3& out) {
+    V3 edge_cross = normalized(cross(edge, tri_normal));
+    float cur = get_angle(edge_cross, tri_normal, n);
+    if (corrected < 0) {
+        if (cur < corrected) {
+            out = m3_from_quat(q_axis_angle(edge, corrected - cur)) * n;
+            return true;
+        }
+    }
+    if (corrected >= 0) {
+        if (cur > corrected) {
+            out = m3_from_quat(q_axis_angle(edge, corrected - cur)) * n;
+            return true;
+        }
+    }
+    return false;
+}
+
+RSX_HD void adjust_internal_edge(const MeshView& mv, int tri, Contact& cp) {
+    const float* info = mv.tri_info + (size_t)tri * 4;
+    uint32_t flags = __float_as_uint_rsx(info[3]);
+    const float angles[3] = {info[0], info[1], info[2]};
+    const float kMaxEdgeAngle = 2.0f * kPi, kEdgeDist = 0.1f, kConvexEps = 0.f;
+    if (!(fabsf(angles[0]) < kMaxEdgeAngle) && !(fabsf(angles[1]) < kMaxEdgeAngle) && !(fabsf(angles[2]) < kMaxEdgeAngle))
+        return;
+    V3 v[3];
+    load_tri(mv, tri, v[0], v[1], v[2]);
+    V3 tri_normal = normalized(cross(v[1] - v[0], v[2] - v[0]));
+    V3 contact = cp.localB;
+    V3 local_n = normalized(transpose(m3_identity()) * cp.normal);
+    int best = -1;
+    float best_d = kLargeFloat;
+    // edges: 0 = v0v1, 1 = v1v2, 2 = v2v0
+    const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
+    for (int e = 0; e < 3; e++) {
+        if (fabsf(angles[e]) < kMaxEdgeAngle) {
+            V3 nr;
+            nearest_on_segment(cp.localB, v[ea[e]], v[eb[e]], nr);
+            float l = len(contact - nr);
+            if (l < best_d) {
+                best = e;
+                best_d = l;
+            }
+        }
+    }
+    bool near_edge = false;
+    int concave_hits = 0;
+    const uint32_t convex_bit[3] = {1, 2, 4}, swap_bit[3] = {8, 16, 32};
+    for (int e = 0; e < 3; e++) {
+        V3 nr;
+        nearest_on_segment(contact, v[ea[e]], v[eb[e]], nr);
+        if (!(fabsf(angles[e]) < kMaxEdgeAngle)) continue;
+        float l = len(contact - nr);
+        if (!(l < kEdgeDist) || best != e) continue;
+        V3 edge = v[ea[e]] - v[eb[e]];
+        near_edge = true;
+        if (angles[e] == 0.f) {
+            concave_hits++;
+            continue;
+        }
+        bool convex = (flags & convex_bit[e]) != 0;
+        float sf = convex ? 1.f : -1.f;
+        V3 nA = sf * tri_normal;
+        V3 cnb = quat_rotate_v(q_axis_angle(edge, angles[e]), tri_normal);
+        if (flags & swap_bit[e]) cnb = cnb * -1.f;
+        V3 nB = sf * cnb;
+        float da = dot(local_n, nA), db = dot(local_n, nB);
+        bool back = (da < kConvexEps) && (db < kConvexEps);
+        if (back) {
+            concave_hits++;
+        } else {
+            V3 ln = (e == 0) ? local_n : (transpose(m3_identity()) * cp.normal);
+            V3 clamped;
+            if (clamp_normal(edge, sf * tri_normal, ln, angles[e], clamped)) {
+                if (dot(clamped, 1.f * tri_normal) > 0) {
+                    cp.normal = m3_identity() * clamped;
+                    cp.posB = cp.posA - cp.normal * cp.dist;
+                    cp.localB = inv_xform(xf_identity(), cp.posB);
+                }
+            }
+        }
+    }
+    if (near_edge && concave_hits > 0) {
+        V3 nn = tri_normal * 1.f;
+        float d = dot(nn, local_n);
+        if (d < 0) return;
+        cp.normal = m3_identity() * nn;
+        cp.posB = cp.posA - cp.normal * cp.dist;
+        cp.localB = inv_xform(xf_identity(), cp.posB);
+    }
 }
 
 // --- sphere vs plane / triangle ------------------------------------------------

@@ -29,6 +29,7 @@ MeshView HostWorld::view() const {
     v.susp_world = nullptr;
     v.bp_meshes = bp_meshes.data();
     v.mesh_aabb = mesh_aabb.data();
+    v.tri_info = tri_info.data();
     v.grid_min = grid_min;
     v.grid_cell = grid_cell;
     v.gx = gx;
@@ -163,6 +164,8 @@ static void pad_geometry(PadGeom* pads) {
     }
 }
 
+static void internal_edge_info(HostWorld& w);
+
 bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, std::string& err) {
     if (files.size() != (size_t)kNumMeshes) {
         err = "expected 16 collision mesh files";
@@ -296,10 +299,127 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
                             }
                 }
     }
+    internal_edge_info(w);
     car_geometry(w.cg);
     ball_geometry(w.bg);
     pad_geometry(w.pads);
     return true;
+}
+
+
+// --- internal edge information (btGenerateInternalEdgeInfo) --------------------
+// For every triangle and each of its edges shared with another triangle of the
+// same mesh, the signed angle to the neighbour and convexity flags, computed
+// with the steps of btConnectivityProcessor::processTriangle
+// (btInternalEdgeUtility.cpp:46-250).
+static float bt_get_angle(V3 edge_a, V3 normal_a, V3 normal_b) {
+    return atan2f(dot(normal_b, edge_a), dot(normal_b, normal_a));
+}
+static V3 quat_rotate(Q4 r, V3 v) {
+    // q = r * v (pure quaternion), q *= r^-1
+    Q4 q = Q4{r.w * v.x + r.y * v.z - r.z * v.y, r.w * v.y + r.z * v.x - r.x * v.z, r.w * v.z + r.x * v.y - r.y * v.x,
+              -r.x * v.x - r.y * v.y - r.z * v.z};
+    q = qmul(q, qinverse(r));
+    return V3{q.x, q.y, q.z};
+}
+
+static void connectivity(const V3* A, const V3* B, float* info) {
+    const float equal_thr = 0.0001f * 0.0001f;
+    const float planar_eps = 0.0001f;
+    int numshared = 0;
+    int sa[3] = {-1, -1, -1}, sb[3] = {-1, -1, -1};
+    if (len2(cross(B[1] - B[0], B[2] - B[0])) < equal_thr) return;
+    if (len2(cross(A[1] - A[0], A[2] - A[0])) < equal_thr) return;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            if (len2(A[i] - B[j]) < equal_thr) {
+                sa[numshared] = i;
+                sb[numshared] = j;
+                numshared++;
+                if (numshared >= 3) return;
+            }
+        }
+        if (numshared >= 3) return;
+    }
+    if (numshared != 2) return;
+    if (sa[0] == 0 && sa[1] == 2) {
+        sa[0] = 2;
+        sa[1] = 0;
+        int t = sb[1];
+        sb[1] = sb[0];
+        sb[0] = t;
+    }
+    int sum_a = sa[0] + sa[1];
+    int other_a = 3 - sum_a;
+    V3 edge = A[sa[1]] - A[sa[0]];
+    int other_b = 3 - (sb[0] + sb[1]);
+    V3 tb[3] = {B[sb[1]], B[sb[0]], B[other_b]};
+    V3 normal_a = normalized(cross(A[1] - A[0], A[2] - A[0]));
+    V3 normal_b = normalized(cross(tb[1] - tb[0], tb[2] - tb[0]));
+    edge = normalized(edge);
+    V3 ecA = normalized(cross(edge, normal_a));
+    if (dot(ecA, A[other_a] - A[sa[0]]) < 0) ecA = ecA * -1.f;
+    V3 ecB = normalized(cross(edge, normal_b));
+    if (dot(ecB, B[other_b] - B[sb[0]]) < 0) ecB = ecB * -1.f;
+    float ang4 = 0.f, corrected = 0.f;
+    bool is_convex = false;
+    V3 calc_edge = cross(ecA, ecB);
+    float l2 = len2(calc_edge);
+    if (!(l2 < planar_eps)) {
+        calc_edge = normalized(calc_edge);
+        V3 calc_na = normalized(cross(calc_edge, ecA));
+        float angle2 = bt_get_angle(calc_na, ecA, ecB);
+        ang4 = kPi - angle2;
+        float dot_a = dot(normal_a, ecB);
+        is_convex = (dot_a < 0.f);
+        corrected = is_convex ? ang4 : -ang4;
+    }
+    uint32_t flags;
+    memcpy(&flags, &info[3], 4);
+    int slot;
+    V3 e;
+    uint32_t swap_bit, convex_bit;
+    if (sum_a == 1) { e = A[0] - A[1]; slot = 0; swap_bit = 8; convex_bit = 1; }
+    else if (sum_a == 2) { e = A[2] - A[0]; slot = 2; swap_bit = 32; convex_bit = 4; }
+    else { e = A[1] - A[2]; slot = 1; swap_bit = 16; convex_bit = 2; }
+    Q4 orn = q_axis_angle(e, -corrected);
+    V3 cnb = quat_rotate(orn, normal_a);
+    if (dot(cnb, normal_b) < 0) flags |= swap_bit;
+    info[slot] = -corrected;
+    if (is_convex) flags |= convex_bit;
+    memcpy(&info[3], &flags, 4);
+}
+
+static void internal_edge_info(HostWorld& w) {
+    const int ntri = (int)w.tri_mesh.size();
+    w.tri_info.assign((size_t)ntri * 4, 0.f);
+    const float two_pi = 2.0f * kPi;
+    for (int t = 0; t < ntri; t++) {
+        w.tri_info[(size_t)t * 4 + 0] = two_pi;
+        w.tri_info[(size_t)t * 4 + 1] = two_pi;
+        w.tri_info[(size_t)t * 4 + 2] = two_pi;
+    }
+    std::vector<uint8_t> has(ntri, 0);
+    MeshView mv = w.view();
+    std::vector<int> cand(4096);
+    for (int t = 0; t < ntri; t++) {
+        V3 A[3];
+        load_tri(mv, t, A[0], A[1], A[2]);
+        V3 mn = vmin(vmin(A[0], A[1]), A[2]), mx = vmax(vmax(A[0], A[1]), A[2]);
+        int n = gather_tris(mv, mn, mx, cand.data(), (int)cand.size());
+        float info[4] = {two_pi, two_pi, two_pi, 0.f};
+        bool any = false;
+        for (int k = 0; k < n; k++) {
+            int u = cand[k];
+            if (u == t || w.tri_mesh[u] != w.tri_mesh[t]) continue;
+            V3 B[3];
+            load_tri(mv, u, B[0], B[1], B[2]);
+            float before[4] = {info[0], info[1], info[2], info[3]};
+            connectivity(A, B, info);
+            if (memcmp(before, info, sizeof(info)) != 0) any = true;
+        }
+        if (any) memcpy(&w.tri_info[(size_t)t * 4], info, sizeof(info));
+    }
 }
 
 // --- record conversion -----------------------------------------------------------

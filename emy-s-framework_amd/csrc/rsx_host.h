@@ -18,6 +18,7 @@ struct HostWorld {
     std::vector<int> cell_start, cell_tris;
     std::vector<uint16_t> bp_meshes;
     std::vector<float> mesh_aabb;  // 16 * 6
+    std::vector<float> tri_info;   // num_tris * 4
     std::vector<uint32_t> mesh_hash;
     V3 grid_min;
     float grid_cell;

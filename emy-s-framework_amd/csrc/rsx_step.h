@@ -68,13 +68,14 @@ RSX_HD Xf body_xf(const Arena<MAXC>& A, int code) {
 
 // Contact-added callback (Arena::_BulletContactAddedCallback, Arena.cpp:218-427)
 template <int MAXC>
-RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx) {
+RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx, int tri = -1) {
     Contact& cp = m.pt[idx];
     int b0 = m.body0, b1 = m.body1;
     bool st1 = b1 >= kBodyStatic;
     if (b0 == 0 && st1) {
-        // ball + world: special resolution (btAdjustInternalEdgeContacts not yet applied)
+        // ball + world: special resolution, then internal-edge correction
         cp.special = true;
+        if (tri >= 0) adjust_internal_edge(T.mv, tri, cp);
         return;
     }
     if (b0 >= 1 && b0 <= MAXC && st1) {
@@ -83,6 +84,7 @@ RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx
         c.world_contact_normal = cp.normal;
         cp.friction = kCarWorldFriction;
         cp.restitution = kCarWorldRestitution;
+        if (tri >= 0) adjust_internal_edge(T.mv, tri, cp);
         return;
     }
     if (b0 == 0 && b1 >= 1 && b1 <= MAXC) {
@@ -134,7 +136,7 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
         float depth;
         if (sphere_triangle(b.xf.o, T.bg.radius, m.breaking, v0, v1, v2, pt, nrm, depth)) {
             int idx = manifold_add(m, b.xf, trB, nrm, pt, depth, minf(kBallFriction, 0.6f), maxf(kBallRestitution, 0.3f));
-            if (idx >= 0) contact_added(A, T, m, idx);
+            if (idx >= 0) contact_added(A, T, m, idx, t);
         }
     }
     manifold_refresh(m, b.xf, trB);

@@ -55,6 +55,7 @@ struct MeshView {
     const uint8_t* susp_world; // suspension grid cells with world geometry (bled), 32*56*8
     const uint16_t* bp_meshes; // broadphase grid: per cell, mask of meshes listed in the cell
     const float* mesh_aabb;    // [16][6] shape AABB min xyz, max xyz (BT)
+    const float* tri_info;     // [num_tris][4] internal-edge angles v0v1, v1v2, v2v0 and flag bits
     V3 grid_min;               // uniform grid origin (BT)
     float grid_cell;           // cell size (BT)
     int gx, gy, gz;

@@ -173,13 +173,40 @@ def max_err(a, b):
     return float(np.max(np.abs(a - b) / (ABS + REL * np.abs(b))))
 
 
+def vec_err(a, b, abs_=ABS, rel=REL):
+    """Error of a 3-vector quantity in units of its tolerance: the largest
+    component difference over (abs + rel * |reference vector|)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / (abs_ + rel * np.linalg.norm(b)))
+
+
+ROT_ABS = 1e-4  # rotation matrix entries are unitless
+
+
+def phys_errs(av, bv, prefix, fails, rot=True):
+    """av/bv: car_vec (18) or ball_vec (9) arrays; positions in uu, velocities
+    in uu/s, angular velocities in rad/s are each checked as a vector within
+    1e-3 absolute + 1e-4 relative to the reference vector's magnitude."""
+    if len(av) == 18:
+        parts = {"pos": (0, 3), "vel": (12, 15), "ang": (15, 18)}
+    else:
+        parts = {"pos": (0, 3), "vel": (3, 6), "ang": (6, 9)}
+    for k, (lo, hi) in parts.items():
+        e = vec_err(av[lo:hi], bv[lo:hi])
+        if e > 1:
+            fails[f"{prefix}.{k}"] = e
+    if rot and len(av) == 18:
+        e = float(np.max(np.abs(av[3:12] - bv[3:12]))) / ROT_ABS
+        if e > 1:
+            fails[f"{prefix}.rot"] = e
+
+
 def compare_tick(ref, eng, t):
     """Returns a dict of failures for tick t (empty = parity)."""
     fails = {}
     for i, (rc, ec) in enumerate(zip(ref.cars[t], eng.cars[t])):
-        e = max_err(R.car_vec(ec), R.car_vec(rc))
-        if e > 1:
-            fails[f"car{i}.phys"] = e
+        phys_errs(R.car_vec(ec), R.car_vec(rc), f"car{i}", fails)
         for f in R.CAR_FLAG_FIELDS:
             if getattr(rc, f) != getattr(ec, f):
                 fails[f"car{i}.{f}"] = (getattr(rc, f), getattr(ec, f))
@@ -188,9 +215,7 @@ def compare_tick(ref, eng, t):
                 fails[f"car{i}.{f}"] = (getattr(rc, f), getattr(ec, f))
         if list(rc.wheels_with_contact) != list(ec.wheels_with_contact):
             fails[f"car{i}.wheels"] = (list(rc.wheels_with_contact), list(ec.wheels_with_contact))
-    e = max_err(R.ball_vec(eng.balls[t]), R.ball_vec(ref.balls[t]))
-    if e > 1:
-        fails["ball"] = e
+    phys_errs(R.ball_vec(eng.balls[t]), R.ball_vec(ref.balls[t]), "ball", fails)
     for p in range(34):
         a, b = ref.pads[t][p], eng.pads[t][p]
         if a.is_active != b.is_active or a.prev_locked_car_id != b.prev_locked_car_id or not close(b.cooldown, a.cooldown):
