@@ -1,0 +1,8 @@
+"""Batched soccar arena engine for AMD Instinct MI355X (gfx950).
+
+The physics of every arena runs in hand-written HIP kernels (librsx.so, C-ABI
+in include/rsx.h); this package is the host-side mirror of the reference's
+RocketSimCudaBatch interface over that library."""
+from .batch import LIB_PATH, MESH_DIR, EXPORTED, RocketSimBatch, RsxError, load_library  # noqa: F401
+from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, NUM_PADS, RotMat, Vec3,  # noqa: F401
+                      PadStates)

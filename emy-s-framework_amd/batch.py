@@ -1,0 +1,159 @@
+"""Host-side mirror of RocketSimCudaBatch (RocketSimCuda/include/RocketSimCuda.h:
+107-262) over the gfx950 engine library librsx.so.
+
+The method names are the reference's; each call goes straight to the C-ABI of
+include/rsx.h.  There is no CPU fallback: constructing a batch without the
+built library, or without a HIP device, raises."""
+import ctypes as C
+import os
+
+from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, NUM_PADS, PadStates)
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
+MESH_DIR = os.path.join(PKG_DIR, "assets", "soccar")
+
+_VP = C.c_void_p
+_I = C.c_int32
+_SIGS = [
+    ("rsx_last_error", C.c_char_p, []),
+    ("rsx_create", _I, [C.POINTER(Config), C.POINTER(_VP)]),
+    ("rsx_destroy", _I, [_VP]),
+    ("rsx_num_arenas", _I, [_VP, C.POINTER(_I)]),
+    ("rsx_team_size", _I, [_VP, _I, C.POINTER(_I)]),
+    ("rsx_get_car_state", _I, [_VP, _I, _I, C.POINTER(CarState)]),
+    ("rsx_set_car_state", _I, [_VP, _I, _I, C.POINTER(CarState)]),
+    ("rsx_get_ball_state", _I, [_VP, _I, C.POINTER(BallState)]),
+    ("rsx_set_ball_state", _I, [_VP, _I, C.POINTER(BallState)]),
+    ("rsx_get_pad_states", _I, [_VP, _I, C.POINTER(BoostPadState)]),
+    ("rsx_set_pad_states", _I, [_VP, _I, C.POINTER(BoostPadState)]),
+    ("rsx_get_arena_info", _I, [_VP, _I, C.POINTER(ArenaInfo)]),
+    ("rsx_set_tick_count", _I, [_VP, _I, C.c_uint64]),
+    ("rsx_reset_arena", _I, [_VP, _I]),
+    ("rsx_reset_all_arenas", _I, [_VP, _VP]),
+    ("rsx_set_all_controls", _I, [_VP, C.POINTER(CarControls), _I]),
+    ("rsx_step_ticks", _I, [_VP, _I, _VP]),
+    ("rsx_synchronize", _I, [_VP]),
+]
+EXPORTED = [name for name, _, _ in _SIGS]
+
+_lib = None
+
+
+class RsxError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Loads librsx.so (once) and declares the signatures of include/rsx.h."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RsxError(f"{path} is missing: the gfx950 engine library must be built first "
+                       "(__graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, res, args in _SIGS:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != 0:
+        raise RsxError(f"rsx error {status}: {_lib.rsx_last_error().decode(errors='replace')}")
+
+
+class RocketSimBatch:
+    """N soccar arenas stepped on one GPU.
+
+    team_sizes: int (every arena) or a sequence of per-arena team sizes 1..3;
+    arena a holds 2*T cars, blue in slots 0..T-1, orange in T..2T-1 (the
+    reference's AddCar order for a full arena)."""
+
+    def __init__(self, num_arenas, team_sizes=1, tick_skip=8, seed=0, device=0, mesh_dir=MESH_DIR):
+        self._lib = load_library()
+        cfg = Config()
+        cfg.num_arenas = int(num_arenas)
+        if isinstance(team_sizes, int):
+            cfg.default_team_size = team_sizes
+            self._teams = None
+        else:
+            arr = (C.c_uint8 * num_arenas)(*[int(t) for t in team_sizes])
+            self._teams = arr
+            cfg.team_size = C.cast(arr, C.POINTER(C.c_uint8))
+        cfg.mesh_dir = mesh_dir.encode()
+        cfg.tick_skip = tick_skip
+        cfg.seed = seed
+        cfg.device = device
+        h = _VP()
+        _check(self._lib.rsx_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.num_arenas = int(num_arenas)
+        self.team_size = [self.GetTeamSize(a) for a in range(self.num_arenas)]
+        self.num_cars = sum(2 * t for t in self.team_size)
+
+    def Destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.rsx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.Destroy()
+
+    def GetTeamSize(self, arena):
+        out = _I()
+        _check(self._lib.rsx_team_size(self._h, arena, C.byref(out)))
+        return out.value
+
+    def GetCarState(self, arena, car):
+        s = CarState()
+        _check(self._lib.rsx_get_car_state(self._h, arena, car, C.byref(s)))
+        return s
+
+    def SetCarState(self, arena, car, state):
+        _check(self._lib.rsx_set_car_state(self._h, arena, car, C.byref(state)))
+
+    def GetBallState(self, arena):
+        s = BallState()
+        _check(self._lib.rsx_get_ball_state(self._h, arena, C.byref(s)))
+        return s
+
+    def SetBallState(self, arena, state):
+        _check(self._lib.rsx_set_ball_state(self._h, arena, C.byref(state)))
+
+    def GetBoostPadStates(self, arena):
+        p = PadStates()
+        _check(self._lib.rsx_get_pad_states(self._h, arena, p))
+        return p
+
+    def SetBoostPadStates(self, arena, pads):
+        if len(pads) != NUM_PADS:
+            raise ValueError("need 34 pad states")
+        _check(self._lib.rsx_set_pad_states(self._h, arena, pads))
+
+    def GetArenaInfo(self, arena):
+        s = ArenaInfo()
+        _check(self._lib.rsx_get_arena_info(self._h, arena, C.byref(s)))
+        return s
+
+    def SetTickCount(self, arena, tick):
+        _check(self._lib.rsx_set_tick_count(self._h, arena, tick))
+
+    def ResetArena(self, arena):
+        _check(self._lib.rsx_reset_arena(self._h, arena))
+
+    def ResetAllArenas(self, stream=None):
+        _check(self._lib.rsx_reset_all_arenas(self._h, stream))
+
+    def SetAllCarControls(self, controls):
+        """controls: ctypes array of CarControls, arena-major, 2*T per arena."""
+        _check(self._lib.rsx_set_all_controls(self._h, controls, len(controls)))
+
+    def Step(self, ticks=1, stream=None):
+        _check(self._lib.rsx_step_ticks(self._h, ticks, stream))
+
+    def Synchronize(self):
+        _check(self._lib.rsx_synchronize(self._h))

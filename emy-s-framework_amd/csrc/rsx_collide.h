@@ -144,7 +144,10 @@ struct MfList {
     Manifold live[kMaxLive];
 };
 
+// Returns the pair's index, or -1 when the pair list is full (the pair is
+// then skipped for this tick).
 RSX_HD int mf_open(MfList& L, int b0, int b1, float breaking) {
+    if (L.count >= kMaxManifolds) return -1;
     int k = L.count++;
     L.body0[k] = (int8_t)b0;
     L.body1[k] = (int8_t)b1;
@@ -162,6 +165,7 @@ RSX_HD int mf_open(MfList& L, int b0, int b1, float breaking) {
 }
 // Keep the manifold's storage only if it ended the tick with contacts.
 RSX_HD void mf_close(MfList& L, int k) {
+    if (k < 0) return;
     int s = L.slot[k];
     if (s < 0) return;
     if (L.live[s].n > 0 && s == L.nlive) L.nlive++;

@@ -1,0 +1,505 @@
+// rsx_engine.hip — gfx950 kernels and the C-ABI of include/rsx.h.
+//
+// Work mapping: one lane owns one arena for a whole launch.  A launch loads
+// the arena's words from the field-major SoA buffer (coalesced across the 64
+// lanes of a wavefront), runs the requested ticks entirely in registers and
+// private memory, and stores the words back.  Arenas are grouped into three
+// buckets by team size (1v1, 2v2, 3v3) and each bucket is stepped by its own
+// instantiation of the kernel, so no wavefront mixes car counts.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rsx.h"
+#include "rsx_host.h"
+#include "rsx_soa.h"
+
+using namespace rsx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+rsx_status fail(rsx_status s, const std::string& msg) {
+    g_last_error = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess) return fail(RSX_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct DevWorld {
+    float* tri = nullptr;
+    int* tri_mesh = nullptr;
+    int* tri_rank = nullptr;
+    int* cell_start = nullptr;
+    int* cell_tris = nullptr;
+    uint16_t* bp_meshes = nullptr;
+    float* mesh_aabb = nullptr;
+    float* tri_info = nullptr;
+    PadGeom* pads = nullptr;
+};
+
+}  // namespace
+
+struct Bucket {
+    int maxc = 0;            // car capacity (2, 4, 6)
+    int n = 0;               // arenas in this bucket
+    int words = 0;           // SoA words per arena
+    uint32_t* d_words = nullptr;
+    rsx_controls* d_ctrl = nullptr;  // n * maxc
+    std::vector<int> global;         // bucket-local -> global arena index
+};
+
+struct rsx_engine {
+    int device = 0;
+    int num_arenas = 0;
+    int tick_skip = 8;
+    float no_touch_timeout = 0.f;
+    HostWorld hw;
+    DevWorld dw;
+    MeshView dview;
+    Bucket bucket[3];
+    std::vector<int> arena_bucket, arena_local, arena_team;
+};
+
+namespace {
+
+constexpr uint32_t kFlagGoal = 1u, kFlagGoalOrange = 2u;
+
+// Kickoff reset of one arena column in place: keeps car ids, teams and the
+// carried wheel values (Car::SetState does not touch them), advances the
+// arena's RNG stream, bumps the episode counter.
+template <int MAXC>
+__host__ __device__ void reset_column(const Col& col, const CarGeom& cg) {
+    using L = Layout<MAXC>;
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, cg);
+    arena_reset_kickoff<MAXC>(A, cg);
+    A.tick = 0;
+    pack_arena<MAXC>(col, A);
+    col.set(L::kArenaBase + kArFlags, 0u);
+    col.setf(L::kArenaBase + kArNoTouch, 0.f);
+    col.set(L::kArenaBase + kArEpisode, col.get(L::kArenaBase + kArEpisode) + 1u);
+}
+
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_step(uint32_t* words, int n, const rsx_controls* ctrl, int ticks,
+                                              CarGeom cg, BallGeom bg, MeshView mv, const PadGeom* pads) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    Col col{words + a, n};
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, cg);
+    for (int i = 0; i < A.ncars; i++) {
+        const rsx_controls& c = ctrl[(int64_t)a * MAXC + i];
+        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
+                                     c.handbrake != 0};
+    }
+    TickCtx T;
+    T.cg = cg;
+    T.bg = bg;
+    T.mv = mv;
+    T.pads = pads;
+    T.dt = 1.f / kTickRate;
+    TickScratch<MAXC> W;
+    uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
+    for (int t = 0; t < ticks; t++) {
+        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
+            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
+    }
+    pack_arena<MAXC>(col, A);
+    col.set(Layout<MAXC>::kArenaBase + kArFlags, flags);
+}
+
+// Kickoff reset of the arenas whose mask byte is non-zero (all when mask is null).
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint8_t* mask, CarGeom cg) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    if (mask && !mask[a]) return;
+    Col col{words + a, n};
+    reset_column<MAXC>(col, cg);
+}
+
+template <typename T>
+rsx_status upload(T** dst, const std::vector<T>& src) {
+    HIP_TRY(hipMalloc((void**)dst, std::max<size_t>(1, src.size()) * sizeof(T)));
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return RSX_OK;
+}
+
+int bucket_of_team(int t) { return t - 1; }
+
+// Host-side pack/unpack of one arena through a contiguous staging column.
+template <int MAXC>
+void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, int team, uint32_t seed, int slot_base_id) {
+    Arena<MAXC> A;
+    memset(&A, 0, sizeof(A));
+    A.ncars = 2 * team;
+    A.rng = seed ? seed : 0x9E3779B9u;
+    A.tick = 0;
+    for (int i = 0; i < A.ncars; i++) {
+        A.car[i].team = i < team ? 0 : 1;
+        A.car[i].id = (uint32_t)(slot_base_id + i + 1);
+    }
+    for (int i = 0; i < A.ncars; i++) car_set_spawn<MAXC>(A.car[i], g, 0, 0, kCarSpawnRestZ, 0, kBoostSpawnAmount);
+    colbuf.assign(Layout<MAXC>::kWords, 0);
+    Col c{colbuf.data(), 1};
+    pack_arena<MAXC>(c, A);
+    c.set(Layout<MAXC>::kArenaBase + kArNcars, (uint32_t)A.ncars);
+    reset_column<MAXC>(c, g);
+    c.set(Layout<MAXC>::kArenaBase + kArEpisode, 0u);
+}
+
+rsx_status column_get(rsx_engine* e, int arena, std::vector<uint32_t>& out, Bucket*& b, int& local) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (arena < 0 || arena >= e->num_arenas) return fail(RSX_INVALID_ARG, "arena index out of range");
+    b = &e->bucket[e->arena_bucket[arena]];
+    local = e->arena_local[arena];
+    out.resize(b->words);
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpy2D(out.data(), sizeof(uint32_t), b->d_words + local, sizeof(uint32_t) * b->n, sizeof(uint32_t),
+                        b->words, hipMemcpyDeviceToHost));
+    return RSX_OK;
+}
+
+rsx_status column_put(rsx_engine* e, Bucket* b, int local, const std::vector<uint32_t>& in) {
+    HIP_TRY(hipMemcpy2D(b->d_words + local, sizeof(uint32_t) * b->n, in.data(), sizeof(uint32_t), sizeof(uint32_t),
+                        b->words, hipMemcpyHostToDevice));
+    return RSX_OK;
+}
+
+int ball_base(int maxc) { return maxc * kCarWords; }
+int pad_base(int maxc) { return ball_base(maxc) + kBallWords; }
+int arena_base(int maxc) { return pad_base(maxc) + kNumPads * kPadWords; }
+
+}  // namespace
+
+extern "C" {
+
+const char* rsx_last_error(void) { return g_last_error.c_str(); }
+
+rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
+    if (!cfg || !out) return fail(RSX_INVALID_ARG, "null config or output pointer");
+    if (cfg->num_arenas <= 0) return fail(RSX_INVALID_ARG, "num_arenas must be positive");
+    if (!cfg->mesh_dir) return fail(RSX_INVALID_ARG, "mesh_dir is required");
+    std::unique_ptr<rsx_engine> e(new rsx_engine());
+    e->device = cfg->device;
+    e->num_arenas = cfg->num_arenas;
+    e->tick_skip = cfg->tick_skip > 0 ? cfg->tick_skip : 8;
+    e->no_touch_timeout = cfg->no_touch_timeout_s;
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(RSX_HIP_ERROR, "no HIP device is available");
+    if (e->device < 0 || e->device >= nd) return fail(RSX_INVALID_ARG, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(e->device));
+    std::string err;
+    if (!load_world(cfg->mesh_dir, e->hw, err)) return fail(RSX_MESH_LOAD_FAILED, err);
+    rsx_status s;
+    if ((s = upload(&e->dw.tri, e->hw.tri)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.tri_mesh, e->hw.tri_mesh)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.tri_rank, e->hw.tri_rank)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.cell_start, e->hw.cell_start)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.cell_tris, e->hw.cell_tris)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.bp_meshes, e->hw.bp_meshes)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.mesh_aabb, e->hw.mesh_aabb)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.tri_info, e->hw.tri_info)) != RSX_OK) return s;
+    std::vector<PadGeom> pads(e->hw.pads, e->hw.pads + kNumPads);
+    if ((s = upload(&e->dw.pads, pads)) != RSX_OK) return s;
+    e->dview = e->hw.view();
+    e->dview.tri = e->dw.tri;
+    e->dview.tri_mesh = e->dw.tri_mesh;
+    e->dview.tri_rank = e->dw.tri_rank;
+    e->dview.cell_start = e->dw.cell_start;
+    e->dview.cell_tris = e->dw.cell_tris;
+    e->dview.bp_meshes = e->dw.bp_meshes;
+    e->dview.mesh_aabb = e->dw.mesh_aabb;
+    e->dview.tri_info = e->dw.tri_info;
+
+    // assign arenas to team-size buckets
+    e->arena_bucket.resize(e->num_arenas);
+    e->arena_local.resize(e->num_arenas);
+    e->arena_team.resize(e->num_arenas);
+    for (int a = 0; a < e->num_arenas; a++) {
+        int t = cfg->team_size ? cfg->team_size[a] : cfg->default_team_size;
+        if (t < 1 || t > 3) return fail(RSX_INVALID_ARG, "team size must be 1, 2 or 3");
+        Bucket& b = e->bucket[bucket_of_team(t)];
+        e->arena_bucket[a] = bucket_of_team(t);
+        e->arena_local[a] = (int)b.global.size();
+        e->arena_team[a] = t;
+        b.global.push_back(a);
+    }
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        b.maxc = 2 * (k + 1);
+        b.n = (int)b.global.size();
+        b.words = arena_base(b.maxc) + kArenaWords;
+        if (b.n == 0) continue;
+        HIP_TRY(hipMalloc((void**)&b.d_words, sizeof(uint32_t) * (size_t)b.words * b.n));
+        HIP_TRY(hipMalloc((void**)&b.d_ctrl, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
+        HIP_TRY(hipMemset(b.d_ctrl, 0, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
+        // initial kickoff states, built on the host and uploaded field-major
+        std::vector<uint32_t> soa((size_t)b.words * b.n), col;
+        for (int l = 0; l < b.n; l++) {
+            uint32_t seed = cfg->seed * 2654435761u + (uint32_t)b.global[l] * 0x9E3779B9u + 1u;
+            if (b.maxc == 2) host_reset_arena<2>(col, e->hw.cg, 1, seed, 0);
+            else if (b.maxc == 4) host_reset_arena<4>(col, e->hw.cg, 2, seed, 0);
+            else host_reset_arena<6>(col, e->hw.cg, 3, seed, 0);
+            for (int w = 0; w < b.words; w++) soa[(size_t)w * b.n + l] = col[w];
+        }
+        HIP_TRY(hipMemcpy(b.d_words, soa.data(), soa.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    *out = e.release();
+    return RSX_OK;
+}
+
+rsx_status rsx_destroy(rsx_engine* e) {
+    if (!e) return RSX_OK;
+    (void)hipSetDevice(e->device);
+    for (auto& b : e->bucket) {
+        (void)hipFree(b.d_words);
+        (void)hipFree(b.d_ctrl);
+    }
+    void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
+                    e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
+    for (void* p : ptrs) (void)hipFree(p);
+    delete e;
+    return RSX_OK;
+}
+
+rsx_status rsx_num_arenas(const rsx_engine* e, int32_t* out) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    *out = e->num_arenas;
+    return RSX_OK;
+}
+
+rsx_status rsx_team_size(const rsx_engine* e, int32_t arena, int32_t* out) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (arena < 0 || arena >= e->num_arenas) return fail(RSX_INVALID_ARG, "arena index out of range");
+    *out = e->arena_team[arena];
+    return RSX_OK;
+}
+
+rsx_status rsx_get_car_state(rsx_engine* e, int32_t arena, int32_t car, rsx_car_state* out) {
+    if (!out) return fail(RSX_INVALID_ARG, "null output");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    if (car < 0 || car >= 2 * e->arena_team[arena]) return fail(RSX_INVALID_ARG, "car index out of range");
+    Car k;
+    memset(&k, 0, sizeof(k));
+    unpack_car(Col{col.data(), 1}, car * kCarWords, k, e->hw.cg);
+    car_to_public(k, *out);
+    return RSX_OK;
+}
+
+rsx_status rsx_set_car_state(rsx_engine* e, int32_t arena, int32_t car, const rsx_car_state* in) {
+    if (!in) return fail(RSX_INVALID_ARG, "null input");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    if (car < 0 || car >= 2 * e->arena_team[arena]) return fail(RSX_INVALID_ARG, "car index out of range");
+    Col c{col.data(), 1};
+    Car k;
+    memset(&k, 0, sizeof(k));
+    unpack_car(c, car * kCarWords, k, e->hw.cg);  // keeps id, team and wheel values
+    uint32_t id = k.id;
+    int team = k.team;
+    car_from_public(k, *in, e->hw.cg);
+    k.id = id;
+    k.team = team;
+    pack_car(c, car * kCarWords, k);
+    return column_put(e, b, local, col);
+}
+
+rsx_status rsx_get_ball_state(rsx_engine* e, int32_t arena, rsx_ball_state* out) {
+    if (!out) return fail(RSX_INVALID_ARG, "null output");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    int o = ball_base(b->maxc);
+    out->pos = rsx_vec3{c.getf(o) * kBT2UU, c.getf(o + 1) * kBT2UU, c.getf(o + 2) * kBT2UU};
+    out->vel = rsx_vec3{c.getf(o + 3) * kBT2UU, c.getf(o + 4) * kBT2UU, c.getf(o + 5) * kBT2UU};
+    out->ang_vel = rsx_vec3{c.getf(o + 6), c.getf(o + 7), c.getf(o + 8)};
+    return RSX_OK;
+}
+
+rsx_status rsx_set_ball_state(rsx_engine* e, int32_t arena, const rsx_ball_state* in) {
+    if (!in) return fail(RSX_INVALID_ARG, "null input");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    int o = ball_base(b->maxc);
+    c.setv(o, V3{in->pos.x, in->pos.y, in->pos.z} * kUU2BT);
+    c.setv(o + 3, V3{in->vel.x, in->vel.y, in->vel.z} * kUU2BT);
+    c.setv(o + 6, V3{in->ang_vel.x, in->ang_vel.y, in->ang_vel.z});
+    return column_put(e, b, local, col);
+}
+
+rsx_status rsx_get_pad_states(rsx_engine* e, int32_t arena, rsx_pad_state* out) {
+    if (!out) return fail(RSX_INVALID_ARG, "null output");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    for (int p = 0; p < kNumPads; p++) {
+        int o = pad_base(b->maxc) + p * kPadWords;
+        memset(&out[p], 0, sizeof(out[p]));
+        out[p].is_active = c.get(o) != 0;
+        out[p].cooldown = c.getf(o + 1);
+        out[p].prev_locked_car_id = c.get(o + 2);
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_set_pad_states(rsx_engine* e, int32_t arena, const rsx_pad_state* in) {
+    if (!in) return fail(RSX_INVALID_ARG, "null input");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    for (int p = 0; p < kNumPads; p++) {
+        int o = pad_base(b->maxc) + p * kPadWords;
+        c.set(o, in[p].is_active ? 1u : 0u);
+        c.setf(o + 1, in[p].cooldown);
+        c.set(o + 2, in[p].prev_locked_car_id);
+    }
+    return column_put(e, b, local, col);
+}
+
+rsx_status rsx_get_arena_info(rsx_engine* e, int32_t arena, rsx_arena_info* out) {
+    if (!out) return fail(RSX_INVALID_ARG, "null output");
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    int o = arena_base(b->maxc);
+    memset(out, 0, sizeof(*out));
+    out->tick_count = c.get64(o + kArTickLo);
+    out->num_cars = (int32_t)c.get(o + kArNcars);
+    out->team_size = e->arena_team[arena];
+    out->no_touch_time = c.getf(o + kArNoTouch);
+    out->episode = c.get(o + kArEpisode);
+    uint32_t f = c.get(o + kArFlags);
+    out->goal_scored = (f & kFlagGoal) ? 1 : 0;
+    out->goal_team = (f & kFlagGoal) ? ((f & kFlagGoalOrange) ? 1 : 0) : -1;
+    return RSX_OK;
+}
+
+rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena) {
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    if (b->maxc == 2) reset_column<2>(c, e->hw.cg);
+    else if (b->maxc == 4) reset_column<4>(c, e->hw.cg);
+    else reset_column<6>(c, e->hw.cg);
+    return column_put(e, b, local, col);
+}
+
+rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        dim3 grid((b.n + 63) / 64), block(64);
+        if (b.maxc == 2) hipLaunchKernelGGL(k_reset<2>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
+        else if (b.maxc == 4) hipLaunchKernelGGL(k_reset<4>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
+        else hipLaunchKernelGGL(k_reset<6>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
+        HIP_TRY(hipGetLastError());
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_set_tick_count(rsx_engine* e, int32_t arena, uint64_t tick) {
+    std::vector<uint32_t> col;
+    Bucket* b;
+    int local;
+    rsx_status s = column_get(e, arena, col, b, local);
+    if (s != RSX_OK) return s;
+    Col c{col.data(), 1};
+    c.set64(arena_base(b->maxc) + kArTickLo, tick);
+    return column_put(e, b, local, col);
+}
+
+rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int32_t count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!controls) return fail(RSX_INVALID_ARG, "null controls");
+    int total = 0;
+    for (int a = 0; a < e->num_arenas; a++) total += 2 * e->arena_team[a];
+    if (count != total) return fail(RSX_INVALID_ARG, "controls count must equal the number of cars");
+    HIP_TRY(hipSetDevice(e->device));
+    std::vector<rsx_controls> per[3];
+    for (int k = 0; k < 3; k++) per[k].assign((size_t)e->bucket[k].n * e->bucket[k].maxc, rsx_controls{});
+    int o = 0;
+    for (int a = 0; a < e->num_arenas; a++) {
+        int k = e->arena_bucket[a], l = e->arena_local[a], nc = 2 * e->arena_team[a];
+        for (int i = 0; i < nc; i++) per[k][(size_t)l * e->bucket[k].maxc + i] = controls[o + i];
+        o += nc;
+    }
+    for (int k = 0; k < 3; k++)
+        if (e->bucket[k].n)
+            HIP_TRY(hipMemcpy(e->bucket[k].d_ctrl, per[k].data(), per[k].size() * sizeof(rsx_controls),
+                              hipMemcpyHostToDevice));
+    return RSX_OK;
+}
+
+rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (ticks < 0) return fail(RSX_INVALID_ARG, "ticks must be >= 0");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    const int bs = 64;
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        dim3 grid((b.n + bs - 1) / bs), block(bs);
+        if (b.maxc == 2)
+            hipLaunchKernelGGL(k_step<2>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
+                               e->dview, e->dw.pads);
+        else if (b.maxc == 4)
+            hipLaunchKernelGGL(k_step<4>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
+                               e->dview, e->dw.pads);
+        else
+            hipLaunchKernelGGL(k_step<6>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
+                               e->dview, e->dw.pads);
+        HIP_TRY(hipGetLastError());
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_synchronize(rsx_engine* e) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return RSX_OK;
+}
+
+}  // extern "C"

@@ -2,12 +2,13 @@
 // build of the same code.
 //
 // The arena engine must agree with the CPU RocketSim to 1e-4 relative after a
-// tick, and RocketSim is built on Bullet 3.24's scalar (non-SSE) LinearMath on
-// Linux.  To keep results within rounding of it, every operation here evaluates
-// its terms in the order Bullet's scalar path does (for example a dot product
-// is ((x*x' + y*y') + z*z'), a matrix-vector product is three row dots, a
-// division of a vector by s multiplies by 1/s).  The whole project is compiled
-// with floating-point contraction off so no FMA changes the rounding.
+// tick.  To stay within rounding of Bullet 3.24's LinearMath, every operation
+// here evaluates its terms in the order Bullet's scalar path does (for example
+// a dot product is ((x*x' + y*y') + z*z'), a matrix-vector product is three row
+// dots, a division of a vector by s multiplies by 1/s).  The whole project is
+// compiled with floating-point contraction off so no FMA changes the rounding.
+// (On x86-64 RocketSim's Bullet takes some SSE paths, e.g. normalize() via a
+// refined rsqrt, so agreement is to rounding, not to the bit.)
 //
 // Layouts follow Bullet's conventions: M3 stores rows; the rotation of a body
 // has the body's forward/right/up axes as its COLUMNS; Xf is basis + origin.
@@ -19,6 +20,7 @@
 #include <stdint.h>
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define RSX_HD __host__ __device__ __forceinline__
 #else
 #define RSX_HD inline

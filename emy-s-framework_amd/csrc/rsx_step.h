@@ -248,7 +248,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
             }
             int k = mf_open(L, d, -1, breaking);
-            int s = L.slot[k];
+            int s = k >= 0 ? L.slot[k] : -1;
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
@@ -281,7 +281,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 if (!aabb_overlap(cmn, cmx, pmn, pmx)) continue;
             }
             int k = mf_open(L, d, -1, breaking);
-            int s = L.slot[k];
+            int s = k >= 0 ? L.slot[k] : -1;
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + kNumMeshes + p;
@@ -339,7 +339,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
                 float breaking = minf(T.bg.breaking, g.breaking);
                 int k = mf_open(L, 0, e, breaking);
-                int s = L.slot[k];
+                int s = k >= 0 ? L.slot[k] : -1;
                 if (s >= 0) {
                     Manifold& m = L.live[s];
                     V3 nrm, ptb;
@@ -406,7 +406,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     }
     for (int oi = 0; oi < nm; oi++) {
         int k = order[oi];
-        int s = L.slot[k];
+        int s = k >= 0 ? L.slot[k] : -1;
         if (s < 0) continue;
         const Manifold& m = L.live[s];
         int ia = m.body0;
@@ -654,6 +654,54 @@ RSX_HD void car_respawn(Arena<MAXC>& A, const TickCtx& T, int ci) {
     float sgn = c.team == 0 ? 1.f : -1.f;
     float yw = yaw + (c.team == 0 ? 0.f : 3.14159265358979323846f);
     car_set_spawn<MAXC>(c, T.cg, x, y * sgn, kCarRespawnZ, yw, kBoostSpawnAmount);
+}
+
+}  // namespace rsx
+
+namespace rsx {
+
+// Kickoff reset (Arena::ResetToRandomKickoff, Arena.cpp:112-216) with the
+// engine's own per-arena RNG for the order of the five spawn spots; cars take
+// the CarState defaults (Car.h:14-115) and keep their carried wheel values,
+// as Car::SetState does.
+template <int MAXC>
+RSX_HD void arena_reset_kickoff(Arena<MAXC>& A, const CarGeom& g) {
+    const float sx[5] = {-2048, 2048, -256, 256, 0};
+    const float sy[5] = {-2560, -2560, -3840, -3840, -4608};
+    const float syaw[5] = {(float)(0.78539816339744830962 * 1), (float)(0.78539816339744830962 * 3),
+                           (float)(0.78539816339744830962 * 2), (float)(0.78539816339744830962 * 2),
+                           (float)(0.78539816339744830962 * 2)};
+    int order[5] = {0, 1, 2, 3, 4};
+    for (int i = 4; i > 0; i--) {
+        int j = (int)(rng_next(A.rng) % (uint32_t)(i + 1));
+        int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+    }
+    int nb = 0, no = 0;
+    for (int i = 0; i < A.ncars; i++) {
+        Car& c = A.car[i];
+        int slot = (c.team == 0) ? nb++ : no++;
+        float x, y, yaw;
+        if (slot < 5) {
+            x = sx[order[slot]];
+            y = sy[order[slot]];
+            yaw = syaw[order[slot]];
+        } else {
+            respawn_spot((slot - 5) % 4, x, y, yaw);
+        }
+        if (c.team != 0) {
+            x = -x;
+            y = -y;
+            yaw = (float)((double)yaw + 3.14159265358979323846);
+        }
+        car_set_spawn<MAXC>(c, g, x, y, kCarSpawnRestZ, yaw, kBoostSpawnAmount);
+    }
+    A.ball.xf = xf_identity();
+    A.ball.xf.o = V3{0, 0, kBallRestZ} * kUU2BT;
+    A.ball.vel = A.ball.ang = A.ball.force = A.ball.vel_cache = V3{0, 0, 0};
+    A.ball.interp = A.ball.xf;
+    for (int p = 0; p < kNumPads; p++) A.pad[p] = Pad{true, 0.f, 0, 0};
 }
 
 }  // namespace rsx

@@ -1,0 +1,79 @@
+"""ctypes mirrors of the public C records of include/rsx_types.h.
+
+Field meaning and units follow RocketSim's CarState / BallState /
+BoostPadState / CarControls (RocketSim src/Sim/Car/Car.h:14-115,
+src/Sim/Ball/Ball.h:13-40, src/Sim/BoostPad/BoostPad.h:23-35,
+src/Sim/CarControls.h:6-30): positions in uu, velocities in uu/s, angular
+velocities in rad/s, rotations as the forward/right/up axes."""
+import ctypes as C
+
+import numpy as np
+
+NUM_PADS = 34
+MAX_CARS = 6
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+    def np(self):
+        return np.array([self.x, self.y, self.z], dtype=np.float32)
+
+    def set(self, v):
+        self.x, self.y, self.z = (float(v[0]), float(v[1]), float(v[2]))
+
+
+class RotMat(C.Structure):
+    _fields_ = [("forward", Vec3), ("right", Vec3), ("up", Vec3)]
+
+
+class CarControls(C.Structure):
+    _fields_ = [("throttle", C.c_float), ("steer", C.c_float), ("pitch", C.c_float), ("yaw", C.c_float),
+                ("roll", C.c_float), ("jump", C.c_uint8), ("boost", C.c_uint8), ("handbrake", C.c_uint8),
+                ("_pad", C.c_uint8)]
+
+
+class CarState(C.Structure):
+    _fields_ = [("pos", Vec3), ("rot", RotMat), ("vel", Vec3), ("ang_vel", Vec3),
+                ("boost", C.c_float), ("time_spent_boosting", C.c_float), ("jump_time", C.c_float),
+                ("flip_time", C.c_float), ("air_time", C.c_float), ("air_time_since_jump", C.c_float),
+                ("supersonic_time", C.c_float), ("handbrake_val", C.c_float), ("auto_flip_timer", C.c_float),
+                ("auto_flip_torque_scale", C.c_float), ("demo_respawn_timer", C.c_float),
+                ("car_contact_cooldown", C.c_float), ("flip_rel_torque", Vec3), ("world_contact_normal", Vec3),
+                ("is_on_ground", C.c_uint8), ("has_jumped", C.c_uint8), ("has_double_jumped", C.c_uint8),
+                ("has_flipped", C.c_uint8), ("is_flipping", C.c_uint8), ("is_jumping", C.c_uint8),
+                ("is_supersonic", C.c_uint8), ("is_auto_flipping", C.c_uint8), ("is_demoed", C.c_uint8),
+                ("world_contact", C.c_uint8), ("ball_hit_valid", C.c_uint8), ("team", C.c_uint8),
+                ("wheels_with_contact", C.c_uint8 * 4), ("id", C.c_uint32), ("car_contact_other_id", C.c_uint32),
+                ("ball_hit_tick", C.c_uint64), ("ball_hit_extra_impulse_tick", C.c_uint64),
+                ("ball_hit_rel_pos", Vec3), ("ball_hit_ball_pos", Vec3), ("ball_hit_extra_vel", Vec3),
+                ("last_controls", CarControls)]
+
+
+class BallState(C.Structure):
+    _fields_ = [("pos", Vec3), ("vel", Vec3), ("ang_vel", Vec3)]
+
+
+class BoostPadState(C.Structure):
+    _fields_ = [("is_active", C.c_uint8), ("_pad", C.c_uint8 * 3), ("cooldown", C.c_float),
+                ("prev_locked_car_id", C.c_uint32)]
+
+
+PadStates = BoostPadState * NUM_PADS
+
+
+class ArenaInfo(C.Structure):
+    _fields_ = [("tick_count", C.c_uint64), ("num_cars", C.c_int32), ("team_size", C.c_int32),
+                ("goal_scored", C.c_uint8), ("_pad", C.c_uint8 * 3), ("goal_team", C.c_int32),
+                ("no_touch_time", C.c_float), ("episode", C.c_uint32)]
+
+
+class Config(C.Structure):
+    _fields_ = [("num_arenas", C.c_int32), ("team_size", C.POINTER(C.c_uint8)), ("default_team_size", C.c_int32),
+                ("mesh_dir", C.c_char_p), ("tick_skip", C.c_int32), ("no_touch_timeout_s", C.c_float),
+                ("seed", C.c_uint32), ("device", C.c_int32)]
+
+
+assert C.sizeof(CarControls) == 24
+assert C.sizeof(CarState) == 248
+assert C.sizeof(ArenaInfo) == 32

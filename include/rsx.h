@@ -1,0 +1,93 @@
+/*
+ * rsx.h — C-ABI of the batched arena engine (librsx.so, gfx950).
+ *
+ * One engine holds N independent soccar arenas (1v1, 2v2 or 3v3 each) in GPU
+ * memory and steps them with hand-written HIP kernels.  Every call returns an
+ * rsx_status; rsx_last_error() gives the message of the last failure on the
+ * calling thread.  One engine must be used from one host thread at a time.
+ *
+ * Each entry point names the reference interface it replaces
+ * (RocketSimCuda/include/RocketSimCuda.h, class rsc::RocketSimCudaBatch).
+ * The records are in rsx_types.h.
+ */
+#ifndef RSX_H
+#define RSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rsx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rsx_engine rsx_engine;
+typedef int32_t rsx_status;
+
+#define RSX_OK 0
+#define RSX_INVALID_ARG 1
+#define RSX_OOM 2
+#define RSX_HIP_ERROR 3
+#define RSX_NOT_INITIALIZED 4
+#define RSX_MESH_LOAD_FAILED 5
+
+/* Engine configuration; replaces BatchConfig (RocketSimCuda.h:169-188). */
+typedef struct rsx_config {
+    int32_t num_arenas;          /* N > 0 */
+    const uint8_t* team_size;    /* per-arena cars per team (1..3); NULL -> default_team_size */
+    int32_t default_team_size;   /* used when team_size is NULL */
+    const char* mesh_dir;        /* folder holding mesh_0.cmf .. mesh_15.cmf */
+    int32_t tick_skip;           /* physics ticks per env step (8) */
+    float no_touch_timeout_s;    /* env done after this long without a touch (0 = off) */
+    uint32_t seed;               /* RNG seed for kickoffs and demo respawns */
+    int32_t device;              /* HIP device ordinal */
+} rsx_config;
+
+const char* rsx_last_error(void);
+
+/* RocketSimCudaBatch::Init + AddCar x (N x cars) + ResetAllArenas
+ * (RocketSimCuda.h:262-283, RocketSimCudaApi.cuh:18-201).  Cars of arena a
+ * take slots 0..T-1 for blue and T..2T-1 for orange (T = team size). */
+rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out);
+/* RocketSimCudaBatch::Destroy (RocketSimCuda.h:266) */
+rsx_status rsx_destroy(rsx_engine* e);
+
+rsx_status rsx_num_arenas(const rsx_engine* e, int32_t* out);
+rsx_status rsx_team_size(const rsx_engine* e, int32_t arena, int32_t* out);
+
+/* State access; replace Get/SetCarState, Get/SetBallState,
+ * Get/SetBoostPadStates, GetArenaInfo (RocketSimCuda.h:350-380). */
+rsx_status rsx_get_car_state(rsx_engine* e, int32_t arena, int32_t car, rsx_car_state* out);
+rsx_status rsx_set_car_state(rsx_engine* e, int32_t arena, int32_t car, const rsx_car_state* in);
+rsx_status rsx_get_ball_state(rsx_engine* e, int32_t arena, rsx_ball_state* out);
+rsx_status rsx_set_ball_state(rsx_engine* e, int32_t arena, const rsx_ball_state* in);
+rsx_status rsx_get_pad_states(rsx_engine* e, int32_t arena, rsx_pad_state* out34);
+rsx_status rsx_set_pad_states(rsx_engine* e, int32_t arena, const rsx_pad_state* in34);
+rsx_status rsx_get_arena_info(rsx_engine* e, int32_t arena, rsx_arena_info* out);
+rsx_status rsx_set_tick_count(rsx_engine* e, int32_t arena, uint64_t tick);
+
+/* Kickoff reset of one arena / of every arena (RocketSimCuda.h:126-129,
+ * RocketSimCudaApi.cuh:203-330 with the pure-kickoff curriculum): tick count 0,
+ * goal flag cleared, cars at a random kickoff permutation drawn from the
+ * arena's own RNG stream, ball at rest at centre, every pad active. */
+rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena);
+rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream);
+
+/* Controls of every car, arena-major, 2*T records per arena; replaces
+ * SetAllCarControls (RocketSimCuda.h:296).  Host pointer. */
+rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int32_t count);
+
+/* Step every arena by `ticks` physics ticks; replaces Step
+ * (RocketSimCuda.h:314, RocketSimCudaApi.cuh:551-571).  Asynchronous on
+ * `stream` (a hipStream_t, NULL = default stream). */
+rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream);
+
+/* Wait for all work queued by this engine; replaces Synchronize. */
+rsx_status rsx_synchronize(rsx_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSX_H */

@@ -110,10 +110,11 @@ typedef struct rsx_arena_info {
     uint64_t tick_count;
     int32_t num_cars;
     int32_t team_size;
-    int32_t goal_team;      /* -1 none; team whose goal was scored on (RS_TEAM_FROM_Y) */
+    uint8_t goal_scored;    /* set once the ball is past a goal line (Arena::IsBallScored); cleared by reset */
+    uint8_t _pad[3];
+    int32_t goal_team;      /* -1 none; 0 = blue scored (ball at +y), 1 = orange scored */
     float no_touch_time;    /* seconds since any car touched the ball */
-    uint32_t episode;       /* number of auto-resets this arena has seen */
-    uint32_t _pad;
+    uint32_t episode;       /* number of resets this arena has seen */
 } rsx_arena_info;
 
 #ifdef __cplusplus

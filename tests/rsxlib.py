@@ -3,61 +3,36 @@ from source into oracle/_ref/librsx_ref.so), the CPU build of the engine's tick
 code (tests/cpu_check/librsx_cpucheck.so), and the record types of
 include/rsx_types.h."""
 import ctypes as C
+import importlib.util
 import os
+import sys
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MESH_DIR = os.path.join(ROOT, "emy-s-framework_amd", "assets", "soccar")
+PKG_DIR = os.path.join(ROOT, "emy-s-framework_amd")
+MESH_DIR = os.path.join(PKG_DIR, "assets", "soccar")
 
 
-class Vec3(C.Structure):
-    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
-
-    def np(self):
-        return np.array([self.x, self.y, self.z], dtype=np.float32)
-
-    def set(self, v):
-        self.x, self.y, self.z = (float(v[0]), float(v[1]), float(v[2]))
-
-
-class RotMat(C.Structure):
-    _fields_ = [("forward", Vec3), ("right", Vec3), ("up", Vec3)]
+def import_engine():
+    """Imports the engine package (its folder name is not an identifier)."""
+    if "emy_s_framework_amd" in sys.modules:
+        return sys.modules["emy_s_framework_amd"]
+    spec = importlib.util.spec_from_file_location("emy_s_framework_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["emy_s_framework_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
 
 
-class Controls(C.Structure):
-    _fields_ = [("throttle", C.c_float), ("steer", C.c_float), ("pitch", C.c_float), ("yaw", C.c_float),
-                ("roll", C.c_float), ("jump", C.c_uint8), ("boost", C.c_uint8), ("handbrake", C.c_uint8),
-                ("_pad", C.c_uint8)]
 
+engine = import_engine()
+Vec3, RotMat, CarState, BallState = engine.Vec3, engine.RotMat, engine.CarState, engine.BallState
+Controls = engine.CarControls
+PadState = engine.BoostPadState
+PADS = engine.PadStates
 
-class CarState(C.Structure):
-    _fields_ = [("pos", Vec3), ("rot", RotMat), ("vel", Vec3), ("ang_vel", Vec3),
-                ("boost", C.c_float), ("time_spent_boosting", C.c_float), ("jump_time", C.c_float),
-                ("flip_time", C.c_float), ("air_time", C.c_float), ("air_time_since_jump", C.c_float),
-                ("supersonic_time", C.c_float), ("handbrake_val", C.c_float), ("auto_flip_timer", C.c_float),
-                ("auto_flip_torque_scale", C.c_float), ("demo_respawn_timer", C.c_float),
-                ("car_contact_cooldown", C.c_float), ("flip_rel_torque", Vec3), ("world_contact_normal", Vec3),
-                ("is_on_ground", C.c_uint8), ("has_jumped", C.c_uint8), ("has_double_jumped", C.c_uint8),
-                ("has_flipped", C.c_uint8), ("is_flipping", C.c_uint8), ("is_jumping", C.c_uint8),
-                ("is_supersonic", C.c_uint8), ("is_auto_flipping", C.c_uint8), ("is_demoed", C.c_uint8),
-                ("world_contact", C.c_uint8), ("ball_hit_valid", C.c_uint8), ("team", C.c_uint8),
-                ("wheels_with_contact", C.c_uint8 * 4), ("id", C.c_uint32), ("car_contact_other_id", C.c_uint32),
-                ("ball_hit_tick", C.c_uint64), ("ball_hit_extra_impulse_tick", C.c_uint64),
-                ("ball_hit_rel_pos", Vec3), ("ball_hit_ball_pos", Vec3), ("ball_hit_extra_vel", Vec3),
-                ("last_controls", Controls)]
-
-
-class BallState(C.Structure):
-    _fields_ = [("pos", Vec3), ("vel", Vec3), ("ang_vel", Vec3)]
-
-
-class PadState(C.Structure):
-    _fields_ = [("is_active", C.c_uint8), ("_pad", C.c_uint8 * 3), ("cooldown", C.c_float),
-                ("prev_locked_car_id", C.c_uint32)]
-
-
-PADS = PadState * 34
 
 CAR_FLOAT_FIELDS = ["boost", "time_spent_boosting", "jump_time", "flip_time", "air_time", "air_time_since_jump",
                     "supersonic_time", "handbrake_val", "auto_flip_timer", "auto_flip_torque_scale",
