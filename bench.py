@@ -1,0 +1,192 @@
+"""Env-step throughput of the gfx950 arena engine on one MI355X.
+
+Workload (default): 65536 soccar 1v1 arenas from the random-state workload
+(emy-s-framework_amd/workloads.py, seed 4), a fresh uniform discrete action
+per car each step (device int32 indices over the 90-entry default table),
+tick-skip 8 at 120 Hz, exact collision.  One step = every arena advances one
+env step (8 physics ticks).  Inputs are resident in HBM when the timed region
+starts.
+
+Prints one JSON line: throughput in env-steps/s (arenas x steps / s), the
+physics kernel's HBM roofline fraction measured with HIP events on the stream
+the kernel runs on, and the reference CPU RocketSim timed on this host."""
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "emy-s-framework_amd")
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one 1v1 env step of the physics path: arena state read
+# once and written once (2 x 1132 B: two cars of 400 B, ball 36 B, 34 pads of
+# 8 B, arena record 24 B) plus one int32 action per car (2 x 4 B).
+ALGO_BYTES_1V1 = 2 * 1132 + 2 * 4
+
+
+def _import_package():
+    if "emy_s_framework_amd" in sys.modules:
+        return sys.modules["emy_s_framework_amd"]
+    spec = importlib.util.spec_from_file_location("emy_s_framework_amd", os.path.join(PKG, "__init__.py"),
+                                                  submodule_search_locations=[PKG])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["emy_s_framework_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(seconds, threads, tick_skip=8):
+    """The reference CPU RocketSim (oracle/_ref/librsx_ref.so) on this host:
+    64 1v1 arenas from kickoffs, random default-table actions, Step(8) per env
+    step, arenas split over `threads` threads; runs for about `seconds`."""
+    path = os.path.join(ROOT, "oracle", "_ref", "librsx_ref.so")
+    if not os.path.exists(path):
+        return None
+    eng = _import_package()
+    lib = C.CDLL(path)
+    lib.ref_init.argtypes = [C.c_char_p]
+    lib.ref_arena_new.restype = C.c_void_p
+    lib.ref_arena_new.argtypes = [C.c_int, C.c_int]
+    lib.ref_arena_free.argtypes = [C.c_void_p]
+    lib.ref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
+    lib.ref_run_random.restype = C.c_int64
+    lib.ref_run_random.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_uint32,
+                                   C.POINTER(eng.CarControls), C.c_int]
+    if lib.ref_init(eng.MESH_DIR.encode()) != 0:
+        raise RuntimeError("oracle mesh load failed")
+    lut = eng.default_action_lut()
+    n_arenas = 64
+    arenas = [lib.ref_arena_new(1, 1) for _ in range(n_arenas)]
+    for i, h in enumerate(arenas):
+        lib.ref_reset_kickoff(h, i)
+    per = [arenas[t::threads] for t in range(threads)]
+    arrs = [(C.c_void_p * len(p))(*p) for p in per]
+    steps = [1]
+    done = [0] * threads
+    per_thread_s = [0.0] * threads
+    ready = threading.Barrier(threads + 1)
+    go = threading.Barrier(threads + 1)
+
+    def work(t):
+        # RocketSim builds per-thread caches on a thread's first step: do that
+        # (and time one step) before the timed region.
+        lib.ref_run_random(arrs[t], len(per[t]), 1, tick_skip, 7 + t, lut, len(lut))
+        t0 = time.perf_counter()
+        lib.ref_run_random(arrs[t], len(per[t]), 2, tick_skip, 77 + t, lut, len(lut))
+        per_thread_s[t] = (time.perf_counter() - t0) / 2
+        ready.wait()
+        go.wait()
+        done[t] = lib.ref_run_random(arrs[t], len(per[t]), steps[0], tick_skip, 1000 + t, lut, len(lut))
+
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for th in ths:
+        th.start()
+    ready.wait()
+    steps[0] = max(2, int(seconds / max(max(per_thread_s), 1e-6)))
+    go.wait()
+    t0 = time.perf_counter()
+    for th in ths:
+        th.join()
+    dt = time.perf_counter() - t0
+    steps = steps[0]
+    for h in arenas:
+        lib.ref_arena_free(h)
+    total = sum(done)
+    return {"value": total / dt, "unit": "env-steps/s", "cores": threads, "kind": "reference",
+            "sample": f"64 1v1 arenas x {steps} env steps (tick-skip 8, random default-table actions, "
+                      f"physics only) on {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--arenas", type=int, default=65536)
+    ap.add_argument("--tick-skip", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the arenas are independent and every GPU box here has one GPU "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    eng = _import_package()
+    from emy_s_framework_amd.workloads import random_field_states
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    batch = eng.RocketSimBatch(args.arenas, 1, tick_skip=args.tick_skip, seed=4, device=0)
+    random_field_states(batch, seed=4)
+    n_cars = batch.num_cars
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    actions = [torch.randint(0, 90, (n_cars,), dtype=torch.int32, device=dev, generator=gen)
+               for _ in range(8)]
+    torch.cuda.synchronize()
+
+    def step(i, ev=None):
+        a = actions[i % len(actions)]
+        batch.SetDiscreteActionsFromDevice(a.data_ptr(), n_cars, sp)
+        if ev is not None:
+            ev[0].record(stream)
+        batch.Step(args.tick_skip, sp)
+        if ev is not None:
+            ev[1].record(stream)
+
+    t0 = time.perf_counter()
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    print(f"# warmup: {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step(i, evs[i])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
+
+    value = args.arenas * args.steps / wall
+    achieved = ALGO_BYTES_1V1 * args.arenas / (kern_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_seconds, max(1, min(16, os.cpu_count() or 1)))
+    out = {
+        "metric": "env-steps/sec @65536 1v1 arenas (neural-SDF cull vs exact), % HBM roofline",
+        "value": value,
+        "unit": "env-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.arenas} soccar 1v1 arenas, random field states (seed 4), uniform random "
+                               f"discrete actions, tick-skip {args.tick_skip}, exact collision, physics step only "
+                               f"(no obs/rewards yet)",
+                   "arenas": args.arenas, "tick_skip": args.tick_skip, "cull": "none (exact collision)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step<2>", "kernel_ms": kern_ms, "algo_bytes_per_env_step": ALGO_BYTES_1V1},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batch.Destroy()
+
+
+if __name__ == "__main__":
+    main()

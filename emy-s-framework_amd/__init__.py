@@ -3,6 +3,7 @@
 The physics of every arena runs in hand-written HIP kernels (librsx.so, C-ABI
 in include/rsx.h); this package is the host-side mirror of the reference's
 RocketSimCudaBatch interface over that library."""
-from .batch import LIB_PATH, MESH_DIR, EXPORTED, RocketSimBatch, RsxError, load_library  # noqa: F401
+from .batch import (LIB_PATH, MESH_DIR, EXPORTED, RocketSimBatch, RsxError, default_action_lut,  # noqa: F401
+                    load_library)
 from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, NUM_PADS, RotMat, Vec3,  # noqa: F401
                       PadStates)

@@ -20,6 +20,7 @@ _SIGS = [
     ("rsx_create", _I, [C.POINTER(Config), C.POINTER(_VP)]),
     ("rsx_destroy", _I, [_VP]),
     ("rsx_num_arenas", _I, [_VP, C.POINTER(_I)]),
+    ("rsx_num_cars", _I, [_VP, C.POINTER(_I)]),
     ("rsx_team_size", _I, [_VP, _I, C.POINTER(_I)]),
     ("rsx_get_car_state", _I, [_VP, _I, _I, C.POINTER(CarState)]),
     ("rsx_set_car_state", _I, [_VP, _I, _I, C.POINTER(CarState)]),
@@ -29,6 +30,17 @@ _SIGS = [
     ("rsx_set_pad_states", _I, [_VP, _I, C.POINTER(BoostPadState)]),
     ("rsx_get_arena_info", _I, [_VP, _I, C.POINTER(ArenaInfo)]),
     ("rsx_set_tick_count", _I, [_VP, _I, C.c_uint64]),
+    ("rsx_get_all_car_states", _I, [_VP, C.POINTER(CarState), _I]),
+    ("rsx_set_all_car_states", _I, [_VP, C.POINTER(CarState), _I]),
+    ("rsx_get_all_ball_states", _I, [_VP, C.POINTER(BallState), _I]),
+    ("rsx_set_all_ball_states", _I, [_VP, C.POINTER(BallState), _I]),
+    ("rsx_get_all_pad_states", _I, [_VP, C.POINTER(BoostPadState), _I]),
+    ("rsx_set_all_pad_states", _I, [_VP, C.POINTER(BoostPadState), _I]),
+    ("rsx_get_all_arena_infos", _I, [_VP, C.POINTER(ArenaInfo), _I]),
+    ("rsx_upload_action_lut", _I, [_VP, C.POINTER(CarControls), _I]),
+    ("rsx_set_actions_from_indices", _I, [_VP, C.POINTER(C.c_int32), _I]),
+    ("rsx_set_actions_device", _I, [_VP, _VP, _I, _VP]),
+    ("rsx_default_action_lut", _I, [C.POINTER(CarControls), _I, C.POINTER(_I)]),
     ("rsx_reset_arena", _I, [_VP, _I]),
     ("rsx_reset_all_arenas", _I, [_VP, _VP]),
     ("rsx_set_all_controls", _I, [_VP, C.POINTER(CarControls), _I]),
@@ -42,6 +54,16 @@ _lib = None
 
 class RsxError(RuntimeError):
     pass
+
+
+def default_action_lut():
+    """RLGym DefaultAction's 90-entry table as built by the library."""
+    lib = load_library()
+    n = _I()
+    _check(lib.rsx_default_action_lut(None, 0, C.byref(n)))
+    out = (CarControls * n.value)()
+    _check(lib.rsx_default_action_lut(out, n.value, C.byref(n)))
+    return out
 
 
 def load_library(path=LIB_PATH):
@@ -93,7 +115,10 @@ class RocketSimBatch:
         self._h = h
         self.num_arenas = int(num_arenas)
         self.team_size = [self.GetTeamSize(a) for a in range(self.num_arenas)]
-        self.num_cars = sum(2 * t for t in self.team_size)
+        n = _I()
+        _check(self._lib.rsx_num_cars(self._h, C.byref(n)))
+        self.num_cars = n.value
+        assert self.num_cars == sum(2 * t for t in self.team_size)
 
     def Destroy(self):
         if getattr(self, "_h", None):
@@ -147,6 +172,48 @@ class RocketSimBatch:
 
     def ResetAllArenas(self, stream=None):
         _check(self._lib.rsx_reset_all_arenas(self._h, stream))
+
+    def GetAllCarStates(self):
+        out = (CarState * self.num_cars)()
+        _check(self._lib.rsx_get_all_car_states(self._h, out, self.num_cars))
+        return out
+
+    def SetAllCarStates(self, states):
+        _check(self._lib.rsx_set_all_car_states(self._h, states, len(states)))
+
+    def GetAllBallStates(self):
+        out = (BallState * self.num_arenas)()
+        _check(self._lib.rsx_get_all_ball_states(self._h, out, self.num_arenas))
+        return out
+
+    def SetAllBallStates(self, states):
+        _check(self._lib.rsx_set_all_ball_states(self._h, states, len(states)))
+
+    def GetAllBoostPadStates(self):
+        out = (BoostPadState * (NUM_PADS * self.num_arenas))()
+        _check(self._lib.rsx_get_all_pad_states(self._h, out, len(out)))
+        return out
+
+    def SetAllBoostPadStates(self, states):
+        _check(self._lib.rsx_set_all_pad_states(self._h, states, len(states)))
+
+    def GetAllArenaInfos(self):
+        out = (ArenaInfo * self.num_arenas)()
+        _check(self._lib.rsx_get_all_arena_infos(self._h, out, self.num_arenas))
+        return out
+
+    def UploadDiscreteActionLut(self, lut):
+        _check(self._lib.rsx_upload_action_lut(self._h, lut, len(lut)))
+
+    def SetDiscreteActionsFromIndices(self, indices):
+        """indices: sequence / int32 numpy array of rsx_num_cars action indices (host)."""
+        import numpy as np
+        arr = np.ascontiguousarray(indices, dtype=np.int32)
+        _check(self._lib.rsx_set_actions_from_indices(self._h, arr.ctypes.data_as(C.POINTER(C.c_int32)), arr.size))
+
+    def SetDiscreteActionsFromDevice(self, device_ptr, count, stream=None):
+        """device_ptr: address of a device int32 array (e.g. tensor.data_ptr())."""
+        _check(self._lib.rsx_set_actions_device(self._h, device_ptr, count, stream))
 
     def SetAllCarControls(self, controls):
         """controls: ctypes array of CarControls, arena-major, 2*T per arena."""

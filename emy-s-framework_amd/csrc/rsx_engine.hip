@@ -55,6 +55,7 @@ struct Bucket {
     int words = 0;           // SoA words per arena
     uint32_t* d_words = nullptr;
     rsx_controls* d_ctrl = nullptr;  // n * maxc
+    int* d_car_off = nullptr;        // n: global index of each arena's first car
     std::vector<int> global;         // bucket-local -> global arena index
 };
 
@@ -67,7 +68,12 @@ struct rsx_engine {
     DevWorld dw;
     MeshView dview;
     Bucket bucket[3];
-    std::vector<int> arena_bucket, arena_local, arena_team;
+    std::vector<int> arena_bucket, arena_local, arena_team, arena_car_off;
+    int num_cars = 0;
+    rsx_controls* d_lut = nullptr;
+    int lut_n = 0;
+    int lut_cap = 0;
+    int32_t* d_idx_stage = nullptr;  // staging for host action indices (num_cars)
 };
 
 namespace {
@@ -129,6 +135,18 @@ __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint
     reset_column<MAXC>(col, cg);
 }
 
+// Discrete action expansion: one lane per car slot of the bucket.
+template <int MAXC>
+__global__ void __launch_bounds__(256) k_expand_actions(const int32_t* idx, const int* car_off, int n,
+                                                        const rsx_controls* lut, int lut_n, rsx_controls* ctrl) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * MAXC) return;
+    const int a = t / MAXC, i = t - a * MAXC;
+    int k = idx[car_off[a] + i];
+    k = k < 0 ? 0 : (k >= lut_n ? lut_n - 1 : k);
+    ctrl[t] = lut[k];
+}
+
 template <typename T>
 rsx_status upload(T** dst, const std::vector<T>& src) {
     HIP_TRY(hipMalloc((void**)dst, std::max<size_t>(1, src.size()) * sizeof(T)));
@@ -180,6 +198,19 @@ rsx_status column_put(rsx_engine* e, Bucket* b, int local, const std::vector<uin
 int ball_base(int maxc) { return maxc * kCarWords; }
 int pad_base(int maxc) { return ball_base(maxc) + kBallWords; }
 int arena_base(int maxc) { return pad_base(maxc) + kNumPads * kPadWords; }
+
+void arena_info_from(const Col& c, int maxc, int team, rsx_arena_info& out) {
+    int o = arena_base(maxc);
+    memset(&out, 0, sizeof(out));
+    out.tick_count = c.get64(o + kArTickLo);
+    out.num_cars = (int32_t)c.get(o + kArNcars);
+    out.team_size = team;
+    out.no_touch_time = c.getf(o + kArNoTouch);
+    out.episode = c.get(o + kArEpisode);
+    uint32_t f = c.get(o + kArFlags);
+    out.goal_scored = (f & kFlagGoal) ? 1 : 0;
+    out.goal_team = (f & kFlagGoal) ? ((f & kFlagGoalOrange) ? 1 : 0) : -1;
+}
 
 }  // namespace
 
@@ -236,12 +267,31 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         e->arena_team[a] = t;
         b.global.push_back(a);
     }
+    e->arena_car_off.resize(e->num_arenas);
+    for (int a = 0; a < e->num_arenas; a++) {
+        e->arena_car_off[a] = e->num_cars;
+        e->num_cars += 2 * e->arena_team[a];
+    }
+    {
+        std::vector<rsx_controls> lut = default_action_lut();
+        e->lut_cap = 1024;
+        HIP_TRY(hipMalloc((void**)&e->d_lut, sizeof(rsx_controls) * e->lut_cap));
+        HIP_TRY(hipMemcpy(e->d_lut, lut.data(), lut.size() * sizeof(rsx_controls), hipMemcpyHostToDevice));
+        e->lut_n = (int)lut.size();
+        HIP_TRY(hipMalloc((void**)&e->d_idx_stage, sizeof(int32_t) * (size_t)e->num_cars));
+    }
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
         b.maxc = 2 * (k + 1);
         b.n = (int)b.global.size();
         b.words = arena_base(b.maxc) + kArenaWords;
         if (b.n == 0) continue;
+        {
+            std::vector<int> off(b.n);
+            for (int l = 0; l < b.n; l++) off[l] = e->arena_car_off[b.global[l]];
+            rsx_status st = upload(&b.d_car_off, off);
+            if (st != RSX_OK) return st;
+        }
         HIP_TRY(hipMalloc((void**)&b.d_words, sizeof(uint32_t) * (size_t)b.words * b.n));
         HIP_TRY(hipMalloc((void**)&b.d_ctrl, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
         HIP_TRY(hipMemset(b.d_ctrl, 0, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
@@ -266,7 +316,10 @@ rsx_status rsx_destroy(rsx_engine* e) {
     for (auto& b : e->bucket) {
         (void)hipFree(b.d_words);
         (void)hipFree(b.d_ctrl);
+        (void)hipFree(b.d_car_off);
     }
+    (void)hipFree(e->d_lut);
+    (void)hipFree(e->d_idx_stage);
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
     for (void* p : ptrs) (void)hipFree(p);
@@ -277,6 +330,12 @@ rsx_status rsx_destroy(rsx_engine* e) {
 rsx_status rsx_num_arenas(const rsx_engine* e, int32_t* out) {
     if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
     *out = e->num_arenas;
+    return RSX_OK;
+}
+
+rsx_status rsx_num_cars(const rsx_engine* e, int32_t* out) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    *out = e->num_cars;
     return RSX_OK;
 }
 
@@ -395,17 +454,7 @@ rsx_status rsx_get_arena_info(rsx_engine* e, int32_t arena, rsx_arena_info* out)
     int local;
     rsx_status s = column_get(e, arena, col, b, local);
     if (s != RSX_OK) return s;
-    Col c{col.data(), 1};
-    int o = arena_base(b->maxc);
-    memset(out, 0, sizeof(*out));
-    out->tick_count = c.get64(o + kArTickLo);
-    out->num_cars = (int32_t)c.get(o + kArNcars);
-    out->team_size = e->arena_team[arena];
-    out->no_touch_time = c.getf(o + kArNoTouch);
-    out->episode = c.get(o + kArEpisode);
-    uint32_t f = c.get(o + kArFlags);
-    out->goal_scored = (f & kFlagGoal) ? 1 : 0;
-    out->goal_team = (f & kFlagGoal) ? ((f & kFlagGoalOrange) ? 1 : 0) : -1;
+    arena_info_from(Col{col.data(), 1}, b->maxc, e->arena_team[arena], *out);
     return RSX_OK;
 }
 
@@ -469,6 +518,199 @@ rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int
             HIP_TRY(hipMemcpy(e->bucket[k].d_ctrl, per[k].data(), per[k].size() * sizeof(rsx_controls),
                               hipMemcpyHostToDevice));
     return RSX_OK;
+}
+
+rsx_status rsx_upload_action_lut(rsx_engine* e, const rsx_controls* lut, int32_t count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!lut || count <= 0 || count > e->lut_cap) return fail(RSX_INVALID_ARG, "action table must hold 1..1024 entries");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpy(e->d_lut, lut, sizeof(rsx_controls) * count, hipMemcpyHostToDevice));
+    e->lut_n = count;
+    return RSX_OK;
+}
+
+rsx_status rsx_default_action_lut(rsx_controls* out, int32_t capacity, int32_t* count) {
+    std::vector<rsx_controls> lut = default_action_lut();
+    if (count) *count = (int32_t)lut.size();
+    if (!out) return RSX_OK;
+    if (capacity < (int32_t)lut.size()) return fail(RSX_INVALID_ARG, "capacity below the table size");
+    memcpy(out, lut.data(), lut.size() * sizeof(rsx_controls));
+    return RSX_OK;
+}
+
+rsx_status rsx_set_actions_device(rsx_engine* e, const int32_t* device_indices, int32_t count, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!device_indices) return fail(RSX_INVALID_ARG, "null indices");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "index count must equal the number of cars");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        const int threads = b.n * b.maxc;
+        dim3 grid((threads + 255) / 256), block(256);
+        if (b.maxc == 2)
+            hipLaunchKernelGGL(k_expand_actions<2>, grid, block, 0, st, device_indices, b.d_car_off, b.n, e->d_lut,
+                               e->lut_n, b.d_ctrl);
+        else if (b.maxc == 4)
+            hipLaunchKernelGGL(k_expand_actions<4>, grid, block, 0, st, device_indices, b.d_car_off, b.n, e->d_lut,
+                               e->lut_n, b.d_ctrl);
+        else
+            hipLaunchKernelGGL(k_expand_actions<6>, grid, block, 0, st, device_indices, b.d_car_off, b.n, e->d_lut,
+                               e->lut_n, b.d_ctrl);
+        HIP_TRY(hipGetLastError());
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_set_actions_from_indices(rsx_engine* e, const int32_t* indices, int32_t count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!indices) return fail(RSX_INVALID_ARG, "null indices");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "index count must equal the number of cars");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpy(e->d_idx_stage, indices, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+    rsx_status s = rsx_set_actions_device(e, e->d_idx_stage, count, nullptr);
+    if (s != RSX_OK) return s;
+    HIP_TRY(hipDeviceSynchronize());
+    return RSX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Whole-bucket copies between the device SoA and a host mirror.
+rsx_status bucket_download(rsx_engine* e, int k, std::vector<uint32_t>& host) {
+    Bucket& b = e->bucket[k];
+    host.resize((size_t)b.words * b.n);
+    if (b.n) HIP_TRY(hipMemcpy(host.data(), b.d_words, host.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return RSX_OK;
+}
+
+rsx_status bucket_upload(rsx_engine* e, int k, const std::vector<uint32_t>& host) {
+    Bucket& b = e->bucket[k];
+    if (b.n) HIP_TRY(hipMemcpy(b.d_words, host.data(), host.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return RSX_OK;
+}
+
+// Visits every arena with a host-side column over its bucket's mirror.
+// fn(arena, Col, maxc) ; returns the first error.
+template <typename Fn>
+rsx_status for_all_arenas(rsx_engine* e, bool write_back, Fn fn) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        std::vector<uint32_t> host;
+        rsx_status s = bucket_download(e, k, host);
+        if (s != RSX_OK) return s;
+        for (int l = 0; l < b.n; l++) fn(b.global[l], Col{host.data() + l, b.n}, b.maxc);
+        if (write_back) {
+            s = bucket_upload(e, k, host);
+            if (s != RSX_OK) return s;
+        }
+    }
+    return RSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rsx_status rsx_get_all_car_states(rsx_engine* e, rsx_car_state* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "count must equal the number of cars");
+    return for_all_arenas(e, false, [&](int a, const Col& c, int) {
+        for (int i = 0; i < 2 * e->arena_team[a]; i++) {
+            Car k;
+            memset(&k, 0, sizeof(k));
+            unpack_car(c, i * kCarWords, k, e->hw.cg);
+            car_to_public(k, out[e->arena_car_off[a] + i]);
+        }
+    });
+}
+
+rsx_status rsx_set_all_car_states(rsx_engine* e, const rsx_car_state* in, int32_t count) {
+    if (!e || !in) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "count must equal the number of cars");
+    return for_all_arenas(e, true, [&](int a, const Col& c, int) {
+        for (int i = 0; i < 2 * e->arena_team[a]; i++) {
+            Car k;
+            memset(&k, 0, sizeof(k));
+            unpack_car(c, i * kCarWords, k, e->hw.cg);
+            uint32_t id = k.id;
+            int team = k.team;
+            car_from_public(k, in[e->arena_car_off[a] + i], e->hw.cg);
+            k.id = id;
+            k.team = team;
+            pack_car(c, i * kCarWords, k);
+        }
+    });
+}
+
+rsx_status rsx_get_all_ball_states(rsx_engine* e, rsx_ball_state* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_arenas) return fail(RSX_INVALID_ARG, "count must equal the number of arenas");
+    return for_all_arenas(e, false, [&](int a, const Col& c, int maxc) {
+        Ball b;
+        memset(&b, 0, sizeof(b));
+        int o = ball_base(maxc);
+        b.xf.o = c.getv(o);
+        b.vel = c.getv(o + 3);
+        b.ang = c.getv(o + 6);
+        ball_to_public(b, out[a]);
+    });
+}
+
+rsx_status rsx_set_all_ball_states(rsx_engine* e, const rsx_ball_state* in, int32_t count) {
+    if (!e || !in) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_arenas) return fail(RSX_INVALID_ARG, "count must equal the number of arenas");
+    return for_all_arenas(e, true, [&](int a, const Col& c, int maxc) {
+        Ball b;
+        memset(&b, 0, sizeof(b));
+        ball_from_public(b, in[a]);
+        int o = ball_base(maxc);
+        c.setv(o, b.xf.o);
+        c.setv(o + 3, b.vel);
+        c.setv(o + 6, b.ang);
+    });
+}
+
+rsx_status rsx_get_all_pad_states(rsx_engine* e, rsx_pad_state* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != kNumPads * e->num_arenas) return fail(RSX_INVALID_ARG, "count must be 34 x the number of arenas");
+    return for_all_arenas(e, false, [&](int a, const Col& c, int maxc) {
+        for (int p = 0; p < kNumPads; p++) {
+            int o = pad_base(maxc) + p * kPadWords;
+            rsx_pad_state& r = out[(size_t)a * kNumPads + p];
+            memset(&r, 0, sizeof(r));
+            r.is_active = c.get(o) != 0;
+            r.cooldown = c.getf(o + 1);
+            r.prev_locked_car_id = c.get(o + 2);
+        }
+    });
+}
+
+rsx_status rsx_set_all_pad_states(rsx_engine* e, const rsx_pad_state* in, int32_t count) {
+    if (!e || !in) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != kNumPads * e->num_arenas) return fail(RSX_INVALID_ARG, "count must be 34 x the number of arenas");
+    return for_all_arenas(e, true, [&](int a, const Col& c, int maxc) {
+        for (int p = 0; p < kNumPads; p++) {
+            int o = pad_base(maxc) + p * kPadWords;
+            const rsx_pad_state& r = in[(size_t)a * kNumPads + p];
+            c.set(o, r.is_active ? 1u : 0u);
+            c.setf(o + 1, r.cooldown);
+            c.set(o + 2, r.prev_locked_car_id);
+        }
+    });
+}
+
+rsx_status rsx_get_all_arena_infos(rsx_engine* e, rsx_arena_info* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_arenas) return fail(RSX_INVALID_ARG, "count must equal the number of arenas");
+    return for_all_arenas(e, false, [&](int a, const Col& c, int maxc) { arena_info_from(c, maxc, e->arena_team[a], out[a]); });
 }
 
 rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {

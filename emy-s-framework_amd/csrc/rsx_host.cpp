@@ -442,6 +442,46 @@ rsx_controls controls_to_public(const Controls& c) {
     return r;
 }
 
+// RLGym's DefaultAction table (DefaultAction.cpp:3-59): 24 ground actions
+// (throttle x steer x boost x handbrake, boosting only at full throttle; yaw
+// follows steer), then 66 air actions (pitch x yaw x roll x jump x boost, no
+// yaw with jump, no duplicate of the all-zero ground rows; throttle follows
+// boost, steer follows yaw, handbrake on for jumps with a direction).
+std::vector<rsx_controls> default_action_lut() {
+    std::vector<rsx_controls> out;
+    const float tri[3] = {-1, 0, 1};
+    auto push = [&](float thr, float steer, float pitch, float yaw, float roll, int jump, int boost, int hb) {
+        rsx_controls c{};
+        c.throttle = thr;
+        c.steer = steer;
+        c.pitch = pitch;
+        c.yaw = yaw;
+        c.roll = roll;
+        c.jump = (uint8_t)jump;
+        c.boost = (uint8_t)boost;
+        c.handbrake = (uint8_t)hb;
+        out.push_back(c);
+    };
+    for (float thr : tri)
+        for (float steer : tri)
+            for (int boost = 0; boost < 2; boost++)
+                for (int hb = 0; hb < 2; hb++) {
+                    if (boost && thr != 1) continue;
+                    push(thr, steer, 0, steer, 0, 0, boost, hb);
+                }
+    for (float pitch : tri)
+        for (float yaw : tri)
+            for (float roll : tri)
+                for (int jump = 0; jump < 2; jump++)
+                    for (int boost = 0; boost < 2; boost++) {
+                        if (jump && yaw != 0) continue;
+                        if (pitch == 0 && roll == 0 && jump == 0) continue;
+                        int hb = (jump && (pitch != 0 || yaw != 0 || roll != 0)) ? 1 : 0;
+                        push((float)boost, yaw, pitch, yaw, roll, jump, boost, hb);
+                    }
+    return out;
+}
+
 // Car::SetState (Car.cpp:23-36): transform, velocities (uu -> BT), inertia.
 void car_from_public(Car& c, const rsx_car_state& s, const CarGeom& g) {
     c.xf.o = v_in(s.pos) * kUU2BT;

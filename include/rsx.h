@@ -7,8 +7,12 @@
  * calling thread.  One engine must be used from one host thread at a time.
  *
  * Each entry point names the reference interface it replaces
- * (RocketSimCuda/include/RocketSimCuda.h, class rsc::RocketSimCudaBatch).
- * The records are in rsx_types.h.
+ * (RocketSimCuda/include/RocketSimCuda.h, class rsc::RocketSimCudaBatch,
+ * lines 258-455).  The records are in rsx_types.h.
+ *
+ * Car order: arena-major; arena a holds 2*T cars (T = its team size), blue in
+ * slots 0..T-1 and orange in T..2T-1.  "All cars" arrays therefore have
+ * sum_a 2*T_a entries, with no padding between arenas.
  */
 #ifndef RSX_H
 #define RSX_H
@@ -32,7 +36,7 @@ typedef int32_t rsx_status;
 #define RSX_NOT_INITIALIZED 4
 #define RSX_MESH_LOAD_FAILED 5
 
-/* Engine configuration; replaces BatchConfig (RocketSimCuda.h:169-188). */
+/* Engine configuration; replaces BatchConfig (RocketSimCuda.h:171-188). */
 typedef struct rsx_config {
     int32_t num_arenas;          /* N > 0 */
     const uint8_t* team_size;    /* per-arena cars per team (1..3); NULL -> default_team_size */
@@ -46,18 +50,22 @@ typedef struct rsx_config {
 
 const char* rsx_last_error(void);
 
-/* RocketSimCudaBatch::Init + AddCar x (N x cars) + ResetAllArenas
- * (RocketSimCuda.h:262-283, RocketSimCudaApi.cuh:18-201).  Cars of arena a
- * take slots 0..T-1 for blue and T..2T-1 for orange (T = team size). */
+/* RocketSimCudaBatch::Init + AddCar x (2*T per arena) + ResetAllArenas
+ * (RocketSimCuda.h:264, 272, 278; RocketSimCudaApi.cuh:18-201).  Arenas start
+ * at a kickoff; the default discrete action table (RLGym DefaultAction, 90
+ * entries) is loaded. */
 rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out);
-/* RocketSimCudaBatch::Destroy (RocketSimCuda.h:266) */
+/* RocketSimCudaBatch::Destroy (RocketSimCuda.h:267) */
 rsx_status rsx_destroy(rsx_engine* e);
 
 rsx_status rsx_num_arenas(const rsx_engine* e, int32_t* out);
+/* Total cars over all arenas (the length of every "all cars" array). */
+rsx_status rsx_num_cars(const rsx_engine* e, int32_t* out);
 rsx_status rsx_team_size(const rsx_engine* e, int32_t arena, int32_t* out);
 
-/* State access; replace Get/SetCarState, Get/SetBallState,
- * Get/SetBoostPadStates, GetArenaInfo (RocketSimCuda.h:350-380). */
+/* Per-arena state access; replace GetCarState (RocketSimCuda.h:350),
+ * GetBallState (356), GetBoostPadStates (362), GetArenaInfo (366),
+ * SetCarState (375), SetBallState (376), SetBoostPadStates (378). */
 rsx_status rsx_get_car_state(rsx_engine* e, int32_t arena, int32_t car, rsx_car_state* out);
 rsx_status rsx_set_car_state(rsx_engine* e, int32_t arena, int32_t car, const rsx_car_state* in);
 rsx_status rsx_get_ball_state(rsx_engine* e, int32_t arena, rsx_ball_state* out);
@@ -67,23 +75,47 @@ rsx_status rsx_set_pad_states(rsx_engine* e, int32_t arena, const rsx_pad_state*
 rsx_status rsx_get_arena_info(rsx_engine* e, int32_t arena, rsx_arena_info* out);
 rsx_status rsx_set_tick_count(rsx_engine* e, int32_t arena, uint64_t tick);
 
-/* Kickoff reset of one arena / of every arena (RocketSimCuda.h:126-129,
+/* Bulk state access (synchronous, whole batch); replace GetAllCarStates
+ * (RocketSimCuda.h:353), GetAllBallStates (359), GetAllBoostPadStates (363),
+ * GetAllArenaInfos (367), SetAllBoostPadStates (380).  Counts: cars =
+ * rsx_num_cars, balls and infos = N, pads = 34*N. */
+rsx_status rsx_get_all_car_states(rsx_engine* e, rsx_car_state* out, int32_t count);
+rsx_status rsx_set_all_car_states(rsx_engine* e, const rsx_car_state* in, int32_t count);
+rsx_status rsx_get_all_ball_states(rsx_engine* e, rsx_ball_state* out, int32_t count);
+rsx_status rsx_set_all_ball_states(rsx_engine* e, const rsx_ball_state* in, int32_t count);
+rsx_status rsx_get_all_pad_states(rsx_engine* e, rsx_pad_state* out, int32_t count);
+rsx_status rsx_set_all_pad_states(rsx_engine* e, const rsx_pad_state* in, int32_t count);
+rsx_status rsx_get_all_arena_infos(rsx_engine* e, rsx_arena_info* out, int32_t count);
+
+/* Kickoff reset of one arena / of every arena (RocketSimCuda.h:275, 278;
  * RocketSimCudaApi.cuh:203-330 with the pure-kickoff curriculum): tick count 0,
  * goal flag cleared, cars at a random kickoff permutation drawn from the
  * arena's own RNG stream, ball at rest at centre, every pad active. */
 rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena);
 rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream);
 
-/* Controls of every car, arena-major, 2*T records per arena; replaces
- * SetAllCarControls (RocketSimCuda.h:296).  Host pointer. */
+/* Controls of every car (host array, rsx_num_cars records); replaces
+ * SetAllCarControls (RocketSimCuda.h:291). */
 rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int32_t count);
 
+/* Discrete actions.  rsx_upload_action_lut replaces UploadDiscreteActionLut
+ * (RocketSimCuda.h:294); rsx_set_actions_from_indices replaces
+ * SetDiscreteActionsFromIndices (298, host int32 array); rsx_set_actions_device
+ * replaces SetDiscreteActionsFromDevice (301) and is asynchronous on `stream`.
+ * Indices are clamped into [0, lut size). */
+rsx_status rsx_upload_action_lut(rsx_engine* e, const rsx_controls* lut, int32_t count);
+rsx_status rsx_set_actions_from_indices(rsx_engine* e, const int32_t* indices, int32_t count);
+rsx_status rsx_set_actions_device(rsx_engine* e, const int32_t* device_indices, int32_t count, void* stream);
+/* The default table (RLGym DefaultAction, DefaultAction.cpp:3-89), 90 entries. */
+rsx_status rsx_default_action_lut(rsx_controls* out, int32_t capacity, int32_t* count);
+
 /* Step every arena by `ticks` physics ticks; replaces Step
- * (RocketSimCuda.h:314, RocketSimCudaApi.cuh:551-571).  Asynchronous on
+ * (RocketSimCuda.h:310, RocketSimCudaApi.cuh:551-571).  Asynchronous on
  * `stream` (a hipStream_t, NULL = default stream). */
 rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream);
 
-/* Wait for all work queued by this engine; replaces Synchronize. */
+/* Wait for all work queued by this engine; replaces Synchronize
+ * (RocketSimCuda.h:345). */
 rsx_status rsx_synchronize(rsx_engine* e);
 
 #ifdef __cplusplus

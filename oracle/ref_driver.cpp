@@ -381,3 +381,30 @@ extern "C" void ref_raytest(void* p, const float* a, const float* b, float* out)
     out[5] = cb.m_hitNormalWorld.x(); out[6] = cb.m_hitNormalWorld.y(); out[7] = cb.m_hitNormalWorld.z();
     out[8] = cb.hasHit() ? (float)body_code(h, cb.m_collisionObject) : -1.f;
 }
+
+// CPU baseline loop for the benchmark: `steps` env steps over the arenas
+// [arenas, arenas + n): each step draws one action per car uniformly from the
+// table (xorshift32 stream seeded by `seed`), then runs Arena::Step(tick_skip).
+// A kickoff reset follows a goal, as the env loop does.  Returns the number of
+// env steps done.
+extern "C" int64_t ref_run_random(void** arenas, int n, int steps, int tick_skip, uint32_t seed,
+                                  const rsx_controls* lut, int lut_n) {
+    uint32_t r = seed ? seed : 1u;
+    auto next = [&]() {
+        r ^= r << 13;
+        r ^= r >> 17;
+        r ^= r << 5;
+        return r;
+    };
+    int64_t done = 0;
+    for (int s = 0; s < steps; s++) {
+        for (int a = 0; a < n; a++) {
+            auto* h = (RefArena*)arenas[a];
+            for (Car* c : h->cars) c->controls = to_controls(lut[next() % (uint32_t)lut_n]);
+            h->arena->Step(tick_skip);
+            if (h->arena->IsBallScored()) h->arena->ResetToRandomKickoff((int)(next() & 0x7fffffff));
+            done++;
+        }
+    }
+    return done;
+}

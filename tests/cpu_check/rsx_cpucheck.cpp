@@ -3,6 +3,7 @@
 // plain C so the CPU-side tests can compare it with the oracle without a GPU.
 // Nothing in the engine's product path loads this library.
 #include "../../emy-s-framework_amd/csrc/rsx_host.h"
+#include "../../emy-s-framework_amd/csrc/rsx_soa.h"
 
 #include <cstring>
 #include <string>
@@ -136,6 +137,33 @@ int cc_step(void* p, int ticks) {
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;
     return scored;
+}
+
+// Stores the arena into the engine's SoA column layout and loads it back, as
+// a launch boundary of the GPU kernels does.
+void cc_soa_roundtrip(void* p, int fill) {
+    auto* a = (CcArena*)p;
+    std::vector<uint32_t> col(Layout<6>::kWords, 0);
+    Col c{col.data(), 1};
+    c.set(Layout<6>::kArenaBase + kArNcars, (uint32_t)a->A.ncars);
+    pack_arena<6>(c, a->A);
+    Controls keep[6];
+    for (int i = 0; i < a->A.ncars; i++) keep[i] = a->A.car[i].controls;
+    // fill: seed of the pseudo-random bytes written over the working state and
+    // the tick scratch before the load, so that anything read before it is
+    // written shows up.
+    uint32_t r = (uint32_t)fill * 2654435761u + 12345u;
+    auto scribble = [&](void* dst, size_t n) {
+        uint8_t* b = (uint8_t*)dst;
+        for (size_t i = 0; i < n; i++) {
+            r ^= r << 13; r ^= r >> 17; r ^= r << 5;
+            b[i] = (uint8_t)r;
+        }
+    };
+    scribble(&a->A, sizeof(a->A));
+    scribble(&a->W, sizeof(a->W));
+    unpack_arena<6>(c, a->A, a->world->w.cg);
+    for (int i = 0; i < a->A.ncars; i++) a->A.car[i].controls = keep[i];
 }
 
 // Manifolds with contacts from the last tick, same record layout as the
