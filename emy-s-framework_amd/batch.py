@@ -7,7 +7,8 @@ built library, or without a HIP device, raises."""
 import ctypes as C
 import os
 
-from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, NUM_PADS, PadStates)
+from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, NUM_PADS, PadStates,
+                      CARS_BLUE_FIRST)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
@@ -91,11 +92,13 @@ def _check(status):
 class RocketSimBatch:
     """N soccar arenas stepped on one GPU.
 
-    team_sizes: int (every arena) or a sequence of per-arena team sizes 1..3;
-    arena a holds 2*T cars, blue in slots 0..T-1, orange in T..2T-1 (the
-    reference's AddCar order for a full arena)."""
+    team_sizes: int (every arena) or a sequence of per-arena team sizes 1..3.
+    Arena a holds 2*T cars; car_order CARS_BLUE_FIRST puts blue in slots
+    0..T-1 and orange in T..2T-1, CARS_ALTERNATING alternates blue and orange
+    (CudaEnvSet's AddCar order)."""
 
-    def __init__(self, num_arenas, team_sizes=1, tick_skip=8, seed=0, device=0, mesh_dir=MESH_DIR):
+    def __init__(self, num_arenas, team_sizes=1, tick_skip=8, seed=0, device=0, mesh_dir=MESH_DIR,
+                 car_order=CARS_BLUE_FIRST):
         self._lib = load_library()
         cfg = Config()
         cfg.num_arenas = int(num_arenas)
@@ -110,6 +113,7 @@ class RocketSimBatch:
         cfg.tick_skip = tick_skip
         cfg.seed = seed
         cfg.device = device
+        cfg.car_order = car_order
         h = _VP()
         _check(self._lib.rsx_create(C.byref(cfg), C.byref(h)))
         self._h = h

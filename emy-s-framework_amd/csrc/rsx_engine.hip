@@ -158,14 +158,15 @@ int bucket_of_team(int t) { return t - 1; }
 
 // Host-side pack/unpack of one arena through a contiguous staging column.
 template <int MAXC>
-void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, int team, uint32_t seed, int slot_base_id) {
+void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, int team, uint32_t seed, int slot_base_id,
+                      int car_order) {
     Arena<MAXC> A;
     memset(&A, 0, sizeof(A));
     A.ncars = 2 * team;
     A.rng = seed ? seed : 0x9E3779B9u;
     A.tick = 0;
     for (int i = 0; i < A.ncars; i++) {
-        A.car[i].team = i < team ? 0 : 1;
+        A.car[i].team = car_order == RSX_CARS_ALTERNATING ? (i & 1) : (i < team ? 0 : 1);
         A.car[i].id = (uint32_t)(slot_base_id + i + 1);
     }
     for (int i = 0; i < A.ncars; i++) car_set_spawn<MAXC>(A.car[i], g, 0, 0, kCarSpawnRestZ, 0, kBoostSpawnAmount);
@@ -222,6 +223,8 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     if (!cfg || !out) return fail(RSX_INVALID_ARG, "null config or output pointer");
     if (cfg->num_arenas <= 0) return fail(RSX_INVALID_ARG, "num_arenas must be positive");
     if (!cfg->mesh_dir) return fail(RSX_INVALID_ARG, "mesh_dir is required");
+    if (cfg->car_order != RSX_CARS_BLUE_FIRST && cfg->car_order != RSX_CARS_ALTERNATING)
+        return fail(RSX_INVALID_ARG, "car_order must be RSX_CARS_BLUE_FIRST or RSX_CARS_ALTERNATING");
     std::unique_ptr<rsx_engine> e(new rsx_engine());
     e->device = cfg->device;
     e->num_arenas = cfg->num_arenas;
@@ -299,9 +302,9 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         std::vector<uint32_t> soa((size_t)b.words * b.n), col;
         for (int l = 0; l < b.n; l++) {
             uint32_t seed = cfg->seed * 2654435761u + (uint32_t)b.global[l] * 0x9E3779B9u + 1u;
-            if (b.maxc == 2) host_reset_arena<2>(col, e->hw.cg, 1, seed, 0);
-            else if (b.maxc == 4) host_reset_arena<4>(col, e->hw.cg, 2, seed, 0);
-            else host_reset_arena<6>(col, e->hw.cg, 3, seed, 0);
+            if (b.maxc == 2) host_reset_arena<2>(col, e->hw.cg, 1, seed, 0, cfg->car_order);
+            else if (b.maxc == 4) host_reset_arena<4>(col, e->hw.cg, 2, seed, 0, cfg->car_order);
+            else host_reset_arena<6>(col, e->hw.cg, 3, seed, 0, cfg->car_order);
             for (int w = 0; w < b.words; w++) soa[(size_t)w * b.n + l] = col[w];
         }
         HIP_TRY(hipMemcpy(b.d_words, soa.data(), soa.size() * sizeof(uint32_t), hipMemcpyHostToDevice));

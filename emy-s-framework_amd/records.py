@@ -71,7 +71,11 @@ class ArenaInfo(C.Structure):
 class Config(C.Structure):
     _fields_ = [("num_arenas", C.c_int32), ("team_size", C.POINTER(C.c_uint8)), ("default_team_size", C.c_int32),
                 ("mesh_dir", C.c_char_p), ("tick_skip", C.c_int32), ("no_touch_timeout_s", C.c_float),
-                ("seed", C.c_uint32), ("device", C.c_int32)]
+                ("seed", C.c_uint32), ("device", C.c_int32), ("car_order", C.c_int32)]
+
+
+CARS_BLUE_FIRST = 0
+CARS_ALTERNATING = 1
 
 
 assert C.sizeof(CarControls) == 24

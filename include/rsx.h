@@ -10,8 +10,8 @@
  * (RocketSimCuda/include/RocketSimCuda.h, class rsc::RocketSimCudaBatch,
  * lines 258-455).  The records are in rsx_types.h.
  *
- * Car order: arena-major; arena a holds 2*T cars (T = its team size), blue in
- * slots 0..T-1 and orange in T..2T-1.  "All cars" arrays therefore have
+ * Car order: arena-major; arena a holds 2*T cars (T = its team size) in the
+ * slot order rsx_config.car_order names.  "All cars" arrays therefore have
  * sum_a 2*T_a entries, with no padding between arenas.
  */
 #ifndef RSX_H
@@ -46,7 +46,14 @@ typedef struct rsx_config {
     float no_touch_timeout_s;    /* env done after this long without a touch (0 = off) */
     uint32_t seed;               /* RNG seed for kickoffs and demo respawns */
     int32_t device;              /* HIP device ordinal */
+    int32_t car_order;           /* RSX_CARS_BLUE_FIRST or RSX_CARS_ALTERNATING */
 } rsx_config;
+
+/* Team of each car slot.  BLUE_FIRST: slots 0..T-1 blue, T..2T-1 orange.
+ * ALTERNATING: blue, orange, blue, orange, ... (the AddCar order of
+ * CudaEnvSet.cpp:216-221). */
+#define RSX_CARS_BLUE_FIRST 0
+#define RSX_CARS_ALTERNATING 1
 
 const char* rsx_last_error(void);
 
