@@ -7,8 +7,8 @@ built library, or without a HIP device, raises."""
 import ctypes as C
 import os
 
-from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, NUM_PADS, PadStates,
-                      CARS_BLUE_FIRST)
+from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, EnvConfig, NUM_PADS,
+                      PadStates, CARS_BLUE_FIRST)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
@@ -45,6 +45,15 @@ _SIGS = [
     ("rsx_reset_arena", _I, [_VP, _I]),
     ("rsx_reset_all_arenas", _I, [_VP, _VP]),
     ("rsx_set_all_controls", _I, [_VP, C.POINTER(CarControls), _I]),
+    ("rsx_configure_env", _I, [_VP, C.POINTER(EnvConfig)]),
+    ("rsx_get_env_config", _I, [_VP, C.POINTER(EnvConfig)]),
+    ("rsx_step_env", _I, [_VP, _VP, _VP]),
+    ("rsx_build_obs", _I, [_VP, _VP]),
+    ("rsx_env_buffers", _I, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_int64)]),
+    ("rsx_obs_layout", _I, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _I]),
+    ("rsx_copy_obs", _I, [_VP, C.POINTER(C.c_float), C.c_int64]),
+    ("rsx_copy_rewards", _I, [_VP, C.POINTER(C.c_float), _I]),
+    ("rsx_copy_terminals", _I, [_VP, C.POINTER(C.c_uint8), _I]),
     ("rsx_step_ticks", _I, [_VP, _I, _VP]),
     ("rsx_synchronize", _I, [_VP]),
 ]
@@ -218,6 +227,54 @@ class RocketSimBatch:
     def SetDiscreteActionsFromDevice(self, device_ptr, count, stream=None):
         """device_ptr: address of a device int32 array (e.g. tensor.data_ptr())."""
         _check(self._lib.rsx_set_actions_device(self._h, device_ptr, count, stream))
+
+    # --- env half of the step -------------------------------------------------
+    def ConfigureEnv(self, goal=1.0, concede=-1.0, touch=1.0, vel_ball_to_goal=1.0, vel_player_to_ball=1.0,
+                     no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True):
+        """ConfigureTrainingRewards + ConfigureTrainingTerminals."""
+        c = EnvConfig(goal, concede, touch, vel_ball_to_goal, vel_player_to_ball, no_touch_timeout_s,
+                      int(goal_terminal), int(auto_reset))
+        _check(self._lib.rsx_configure_env(self._h, C.byref(c)))
+
+    def StepEnv(self, device_actions_ptr, stream=None):
+        """Fused env step from a device int32 array of num_cars action indices."""
+        _check(self._lib.rsx_step_env(self._h, device_actions_ptr, stream))
+
+    def BuildObs(self, stream=None):
+        _check(self._lib.rsx_build_obs(self._h, stream))
+
+    def EnvBuffers(self):
+        """(obs ptr, rewards ptr, terminals ptr, obs float count) device addresses."""
+        o, r, t, n = _VP(), _VP(), _VP(), C.c_int64()
+        _check(self._lib.rsx_env_buffers(self._h, C.byref(o), C.byref(r), C.byref(t), C.byref(n)))
+        return o.value, r.value, t.value, n.value
+
+    def ObsLayout(self):
+        import numpy as np
+        off = np.zeros(self.num_cars, dtype=np.int64)
+        ln = np.zeros(self.num_cars, dtype=np.int32)
+        _check(self._lib.rsx_obs_layout(self._h, off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                        ln.ctypes.data_as(C.POINTER(C.c_int32)), self.num_cars))
+        return off, ln
+
+    def CopyBuiltObs(self):
+        import numpy as np
+        n = self.EnvBuffers()[3]
+        out = np.zeros(n, dtype=np.float32)
+        _check(self._lib.rsx_copy_obs(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return out
+
+    def CopyBuiltRewards(self):
+        import numpy as np
+        out = np.zeros(self.num_cars, dtype=np.float32)
+        _check(self._lib.rsx_copy_rewards(self._h, out.ctypes.data_as(C.POINTER(C.c_float)), self.num_cars))
+        return out
+
+    def CopyBuiltTerminals(self):
+        import numpy as np
+        out = np.zeros(self.num_arenas, dtype=np.uint8)
+        _check(self._lib.rsx_copy_terminals(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), self.num_arenas))
+        return out
 
     def SetAllCarControls(self, controls):
         """controls: ctypes array of CarControls, arena-major, 2*T per arena."""

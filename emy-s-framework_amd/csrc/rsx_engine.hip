@@ -16,6 +16,7 @@
 
 #include "../../include/rsx.h"
 #include "rsx_host.h"
+#include "rsx_env.h"
 #include "rsx_soa.h"
 
 using namespace rsx;
@@ -56,6 +57,7 @@ struct Bucket {
     uint32_t* d_words = nullptr;
     rsx_controls* d_ctrl = nullptr;  // n * maxc
     int* d_car_off = nullptr;        // n: global index of each arena's first car
+    int64_t* d_obs_off = nullptr;    // n: float offset of each arena's first obs row
     std::vector<int> global;         // bucket-local -> global arena index
 };
 
@@ -74,6 +76,14 @@ struct rsx_engine {
     int lut_n = 0;
     int lut_cap = 0;
     int32_t* d_idx_stage = nullptr;  // staging for host action indices (num_cars)
+    // env outputs (device, stable for the engine's lifetime)
+    EnvConfig env;
+    float* d_obs = nullptr;          // rows of every car, arena-major, row length 51 + 19 * cars in arena
+    float* d_rew = nullptr;          // num_cars
+    uint8_t* d_term = nullptr;       // num_arenas
+    int64_t obs_floats = 0;
+    std::vector<int64_t> car_obs_off;  // num_cars
+    std::vector<int32_t> car_obs_len;  // num_cars
 };
 
 namespace {
@@ -133,6 +143,97 @@ __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint
     if (mask && !mask[a]) return;
     Col col{words + a, n};
     reset_column<MAXC>(col, cg);
+}
+
+// Fused env step: action indices -> controls, `ticks` physics ticks, then the
+// stub rewards, the terminal conditions, the kickoff auto-reset of terminal
+// arenas and the DefaultObs rows, all in the lane that owns the arena.
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_env_step(uint32_t* words, int n, const int32_t* act, const int* car_off,
+                                                 const rsx_controls* lut, int lut_n, int ticks, CarGeom cg,
+                                                 BallGeom bg, MeshView mv, const PadGeom* pads, EnvConfig E,
+                                                 float* obs, const int64_t* obs_off, float* rew, uint8_t* term) {
+    using L = Layout<MAXC>;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    Col col{words + a, n};
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, cg);
+    float prev[MAXC][kActionFloats];
+    const int c0 = car_off[a];
+    for (int i = 0; i < A.ncars; i++) {
+        int k = act[c0 + i];
+        k = k < 0 ? 0 : (k >= lut_n ? lut_n - 1 : k);
+        const rsx_controls c = lut[k];
+        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
+                                     c.handbrake != 0};
+        prev[i][0] = c.throttle;
+        prev[i][1] = c.steer;
+        prev[i][2] = c.pitch;
+        prev[i][3] = c.yaw;
+        prev[i][4] = c.roll;
+        prev[i][5] = c.jump ? 1.f : 0.f;
+        prev[i][6] = c.boost ? 1.f : 0.f;
+        prev[i][7] = c.handbrake ? 1.f : 0.f;
+    }
+    TickCtx T;
+    T.cg = cg;
+    T.bg = bg;
+    T.mv = mv;
+    T.pads = pads;
+    T.dt = 1.f / kTickRate;
+    TickScratch<MAXC> W;
+    uint32_t flags = col.get(L::kArenaBase + kArFlags);
+    for (int t = 0; t < ticks; t++) {
+        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
+            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
+    }
+    const bool goal = ball_scored_now(A.ball);
+    bool touched[MAXC];
+    bool any_touch = false;
+    for (int i = 0; i < A.ncars; i++) {
+        touched[i] = touched_in_step(A.car[i], A.tick, ticks);
+        any_touch |= touched[i];
+    }
+    float no_touch = col.getf(L::kArenaBase + kArNoTouch);
+    const uint8_t tt = step_terminal(goal, any_touch, (float)ticks * (1.f / kTickRate), no_touch, E);
+    for (int i = 0; i < A.ncars; i++) rew[c0 + i] = stub_reward<MAXC>(A, i, goal, touched[i], E);
+    uint32_t episode = col.get(L::kArenaBase + kArEpisode);
+    if (tt != kNotTerminal && E.auto_reset) {
+        arena_reset_kickoff<MAXC>(A, cg);
+        A.tick = 0;
+        flags = 0;
+        no_touch = 0.f;
+        episode++;
+        for (int i = 0; i < A.ncars; i++)
+            for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
+    }
+    const int row = obs_row_len(A.ncars);
+    float* o = obs + obs_off[a];
+    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    pack_arena<MAXC>(col, A);
+    col.set(L::kArenaBase + kArFlags, flags);
+    col.setf(L::kArenaBase + kArNoTouch, no_touch);
+    col.set(L::kArenaBase + kArEpisode, episode);
+    term[a] = tt;
+}
+
+// DefaultObs rows of the current state with zero previous actions (the
+// observation after a reset).
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_env_obs(uint32_t* words, int n, CarGeom cg, EnvConfig E, float* obs,
+                                                const int64_t* obs_off) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    Col col{words + a, n};
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, cg);
+    float prev[MAXC][kActionFloats];
+    for (int i = 0; i < MAXC; i++)
+        for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
+    const int row = obs_row_len(A.ncars);
+    float* o = obs + obs_off[a];
+    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
 }
 
 // Discrete action expansion: one lane per car slot of the bucket.
@@ -283,6 +384,37 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         e->lut_n = (int)lut.size();
         HIP_TRY(hipMalloc((void**)&e->d_idx_stage, sizeof(int32_t) * (size_t)e->num_cars));
     }
+    // env outputs: obs rows arena-major, one row per car
+    {
+        EnvConfig& E = e->env;
+        E.goal_w = 1.f;
+        E.concede_scale = -1.f;
+        E.touch_w = 1.f;
+        E.vbtg_w = 1.f;
+        E.vptb_w = 1.f;
+        E.no_touch_timeout = cfg->no_touch_timeout_s;
+        E.goal_terminal = 1;
+        E.auto_reset = 1;
+        if (!obs_pad_map(e->hw.pads, E.pad_map)) return fail(RSX_MESH_LOAD_FAILED, "boost pad layout does not match");
+        e->car_obs_off.resize(e->num_cars);
+        e->car_obs_len.resize(e->num_cars);
+        int64_t off = 0;
+        for (int a = 0; a < e->num_arenas; a++) {
+            int nc = 2 * e->arena_team[a];
+            for (int i = 0; i < nc; i++) {
+                e->car_obs_off[e->arena_car_off[a] + i] = off;
+                e->car_obs_len[e->arena_car_off[a] + i] = obs_row_len(nc);
+                off += obs_row_len(nc);
+            }
+        }
+        e->obs_floats = off;
+        HIP_TRY(hipMalloc((void**)&e->d_obs, sizeof(float) * std::max<int64_t>(1, off)));
+        HIP_TRY(hipMalloc((void**)&e->d_rew, sizeof(float) * (size_t)e->num_cars));
+        HIP_TRY(hipMalloc((void**)&e->d_term, (size_t)e->num_arenas));
+        HIP_TRY(hipMemset(e->d_obs, 0, sizeof(float) * std::max<int64_t>(1, off)));
+        HIP_TRY(hipMemset(e->d_rew, 0, sizeof(float) * (size_t)e->num_cars));
+        HIP_TRY(hipMemset(e->d_term, 0, (size_t)e->num_arenas));
+    }
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
         b.maxc = 2 * (k + 1);
@@ -291,8 +423,14 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         if (b.n == 0) continue;
         {
             std::vector<int> off(b.n);
-            for (int l = 0; l < b.n; l++) off[l] = e->arena_car_off[b.global[l]];
+            std::vector<int64_t> ooff(b.n);
+            for (int l = 0; l < b.n; l++) {
+                off[l] = e->arena_car_off[b.global[l]];
+                ooff[l] = e->car_obs_off[off[l]];
+            }
             rsx_status st = upload(&b.d_car_off, off);
+            if (st != RSX_OK) return st;
+            st = upload(&b.d_obs_off, ooff);
             if (st != RSX_OK) return st;
         }
         HIP_TRY(hipMalloc((void**)&b.d_words, sizeof(uint32_t) * (size_t)b.words * b.n));
@@ -320,7 +458,11 @@ rsx_status rsx_destroy(rsx_engine* e) {
         (void)hipFree(b.d_words);
         (void)hipFree(b.d_ctrl);
         (void)hipFree(b.d_car_off);
+        (void)hipFree(b.d_obs_off);
     }
+    (void)hipFree(e->d_obs);
+    (void)hipFree(e->d_rew);
+    (void)hipFree(e->d_term);
     (void)hipFree(e->d_lut);
     (void)hipFree(e->d_idx_stage);
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
@@ -714,6 +856,122 @@ rsx_status rsx_get_all_arena_infos(rsx_engine* e, rsx_arena_info* out, int32_t c
     if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
     if (count != e->num_arenas) return fail(RSX_INVALID_ARG, "count must equal the number of arenas");
     return for_all_arenas(e, false, [&](int a, const Col& c, int maxc) { arena_info_from(c, maxc, e->arena_team[a], out[a]); });
+}
+
+rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!c) return fail(RSX_INVALID_ARG, "null env config");
+    EnvConfig& E = e->env;
+    E.goal_w = c->goal_weight;
+    E.concede_scale = c->concede_scale;
+    E.touch_w = c->touch_weight;
+    E.vbtg_w = c->vel_ball_to_goal_weight;
+    E.vptb_w = c->vel_player_to_ball_weight;
+    E.no_touch_timeout = c->no_touch_timeout_s;
+    E.goal_terminal = c->goal_terminal != 0;
+    E.auto_reset = c->auto_reset != 0;
+    return RSX_OK;
+}
+
+rsx_status rsx_get_env_config(const rsx_engine* e, rsx_env_config* c) {
+    if (!e || !c) return fail(RSX_INVALID_ARG, "null argument");
+    const EnvConfig& E = e->env;
+    c->goal_weight = E.goal_w;
+    c->concede_scale = E.concede_scale;
+    c->touch_weight = E.touch_w;
+    c->vel_ball_to_goal_weight = E.vbtg_w;
+    c->vel_player_to_ball_weight = E.vptb_w;
+    c->no_touch_timeout_s = E.no_touch_timeout;
+    c->goal_terminal = E.goal_terminal;
+    c->auto_reset = E.auto_reset;
+    return RSX_OK;
+}
+
+rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!device_actions) return fail(RSX_INVALID_ARG, "null actions");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        dim3 grid((b.n + 63) / 64), block(64);
+#define RSX_LAUNCH_ENV(M)                                                                                        \
+    hipLaunchKernelGGL(k_env_step<M>, grid, block, 0, st, b.d_words, b.n, device_actions, b.d_car_off, e->d_lut, \
+                       e->lut_n, e->tick_skip, e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->env, e->d_obs,       \
+                       b.d_obs_off, e->d_rew, e->d_term + 0)
+        if (b.maxc == 2) RSX_LAUNCH_ENV(2);
+        else if (b.maxc == 4) RSX_LAUNCH_ENV(4);
+        else RSX_LAUNCH_ENV(6);
+#undef RSX_LAUNCH_ENV
+        HIP_TRY(hipGetLastError());
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_build_obs(rsx_engine* e, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    HIP_TRY(hipSetDevice(e->device));
+    hipStream_t st = (hipStream_t)stream;
+    for (int k = 0; k < 3; k++) {
+        Bucket& b = e->bucket[k];
+        if (!b.n) continue;
+        dim3 grid((b.n + 63) / 64), block(64);
+        if (b.maxc == 2)
+            hipLaunchKernelGGL(k_env_obs<2>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
+        else if (b.maxc == 4)
+            hipLaunchKernelGGL(k_env_obs<4>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
+        else
+            hipLaunchKernelGGL(k_env_obs<6>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
+        HIP_TRY(hipGetLastError());
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_env_buffers(rsx_engine* e, float** obs, float** rewards, uint8_t** terminals, int64_t* obs_floats) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (obs) *obs = e->d_obs;
+    if (rewards) *rewards = e->d_rew;
+    if (terminals) *terminals = e->d_term;
+    if (obs_floats) *obs_floats = e->obs_floats;
+    return RSX_OK;
+}
+
+rsx_status rsx_obs_layout(const rsx_engine* e, int64_t* row_offset, int32_t* row_len, int32_t count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "count must equal the number of cars");
+    for (int i = 0; i < count; i++) {
+        if (row_offset) row_offset[i] = e->car_obs_off[i];
+        if (row_len) row_len[i] = e->car_obs_len[i];
+    }
+    return RSX_OK;
+}
+
+rsx_status rsx_copy_obs(rsx_engine* e, float* out, int64_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->obs_floats) return fail(RSX_INVALID_ARG, "count must equal the obs buffer size");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, e->d_obs, sizeof(float) * count, hipMemcpyDeviceToHost));
+    return RSX_OK;
+}
+
+rsx_status rsx_copy_rewards(rsx_engine* e, float* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_cars) return fail(RSX_INVALID_ARG, "count must equal the number of cars");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, e->d_rew, sizeof(float) * count, hipMemcpyDeviceToHost));
+    return RSX_OK;
+}
+
+rsx_status rsx_copy_terminals(rsx_engine* e, uint8_t* out, int32_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (count != e->num_arenas) return fail(RSX_INVALID_ARG, "count must equal the number of arenas");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, e->d_term, count, hipMemcpyDeviceToHost));
+    return RSX_OK;
 }
 
 rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {

@@ -1,0 +1,182 @@
+// rsx_env.h — the env half of a step: DefaultObs rows, the four stub rewards,
+// the terminal conditions and the auto-reset, evaluated per arena on the
+// post-step state (host and device).
+//
+// Reference behaviour restated here (RLGymCPP, src/RLGymCPP/):
+//   DefaultObs::BuildObs / AddPlayerToObs   ObsBuilders/DefaultObs.cpp:4-54, DefaultObs.h:11-17
+//   InvertPhys                              Gamestates/StateUtil.cpp:3-17
+//   GameState boost-pad order (+ inverted)  Gamestates/GameState.cpp:10-50, 96-109
+//   Player::UpdateFromCar (touch in step)   Gamestates/Player.cpp:8-24
+//   CarState::HasFlipOrJump                 RocketSim/src/Sim/Car/Car.cpp:285-289
+//   GoalReward, VelocityBallToGoalReward,
+//   VelocityPlayerToBallReward, TouchBall   RewardCore/CommonRewards.h:33-99
+//   GoalScoreCondition, NoTouchCondition    TerminalConditions/*.h
+//   reward sum / terminal priority / reset  EnvSet/EnvSet.cpp:169-242, 392-420
+#pragma once
+
+#include "rsx_step.h"
+
+namespace rsx {
+
+// RLGym CommonValues (CommonValues.h:12-40)
+constexpr float kSideWallX = 4096.f, kBackWallY = 5120.f, kCeilingZ = 2044.f, kBackNetY = 6000.f;
+constexpr float kGoalHeightRL = 642.775f;
+constexpr float kRLCarMaxSpeed = 2300.f, kRLCarMaxAngVel = 5.5f, kRLBallMaxSpeed = 6000.f;
+constexpr int kActionFloats = 8;
+constexpr int kPlayerObs = 19;
+constexpr int kObsHead = 9 + kActionFloats + kNumPads;  // ball, previous action, pads
+
+RSX_HD int obs_row_len(int ncars) { return kObsHead + kPlayerObs * ncars; }
+
+// Terminal codes (TerminalCondition.h TerminalType)
+constexpr uint8_t kNotTerminal = 0, kTerminalNormal = 1, kTerminalTruncated = 2;
+
+struct EnvConfig {
+    float goal_w, concede_scale, touch_w, vbtg_w, vptb_w;
+    float no_touch_timeout;   // seconds; <= 0 disables NoTouchCondition
+    int goal_terminal;        // GoalScoreCondition enabled
+    int auto_reset;           // kickoff reset of terminal arenas inside the step
+    int8_t pad_map[kNumPads]; // obs pad slot i -> engine pad index (GameState boostPadIndexMap)
+};
+
+struct StepOut {
+    bool goal_scored;
+    uint8_t terminal;
+};
+
+RSX_HD bool has_flip_or_jump(const Car& c) {
+    return c.on_ground || (!c.has_flipped && !c.has_double_jumped && c.air_time_since_jump < kDoubleJumpMaxDelay);
+}
+
+RSX_HD bool ball_scored_now(const Ball& b) {
+    return fabsf(b.xf.o.y * kBT2UU) > (kGoalThresholdY + kBallRadius);
+}
+
+// Vec::Normalized (MathTypes.h:88-95): divide by the length unless it is ~0.
+RSX_HD V3 rl_normalized(V3 v) {
+    float l2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    float l = l2 > 0 ? sqrtf(l2) : 0.f;
+    if (l > 1.1920929e-07f * 1.1920929e-07f) return V3{v.x / l, v.y / l, v.z / l};
+    return V3{0, 0, 0};
+}
+
+RSX_HD float rl_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// One player's 19-float block (AddPlayerToObs), inverted for orange viewers.
+template <int MAXC>
+RSX_HD void obs_player(const Car& c, bool inv, float* o) {
+    const float px = 1.f / kSideWallX, py = 1.f / kBackWallY, pz = 1.f / kCeilingZ;
+    const float vc = 1.f / kRLCarMaxSpeed, ac = 1.f / kRLCarMaxAngVel;
+    const float s = inv ? -1.f : 1.f;
+    V3 pos = c.xf.o * kBT2UU, vel = c.vel * kBT2UU, ang = c.ang;
+    V3 fwd = col(c.xf.b, 0), up = col(c.xf.b, 2);
+    o[0] = pos.x * s * px;
+    o[1] = pos.y * s * py;
+    o[2] = pos.z * pz;
+    o[3] = fwd.x * s;
+    o[4] = fwd.y * s;
+    o[5] = fwd.z;
+    o[6] = up.x * s;
+    o[7] = up.y * s;
+    o[8] = up.z;
+    o[9] = vel.x * s * vc;
+    o[10] = vel.y * s * vc;
+    o[11] = vel.z * vc;
+    o[12] = ang.x * s * ac;
+    o[13] = ang.y * s * ac;
+    o[14] = ang.z * ac;
+    o[15] = c.boost / 100.f;
+    o[16] = c.on_ground ? 1.f : 0.f;
+    o[17] = has_flip_or_jump(c) ? 1.f : 0.f;
+    o[18] = c.is_demoed ? 1.f : 0.f;
+}
+
+// DefaultObs row of car `self`; prev holds each car's previous action
+// (8 floats per car).  Writes obs_row_len(ncars) floats.
+template <int MAXC>
+RSX_HD void obs_row(const Arena<MAXC>& A, int self, const float (*prev)[kActionFloats], const EnvConfig& E, float* o) {
+    const Car& me = A.car[self];
+    const bool inv = me.team != 0;
+    const float s = inv ? -1.f : 1.f;
+    const float px = 1.f / kSideWallX, py = 1.f / kBackWallY, pz = 1.f / kCeilingZ;
+    const float vc = 1.f / kRLCarMaxSpeed, ac = 1.f / kRLCarMaxAngVel;
+    V3 bp = A.ball.xf.o * kBT2UU, bv = A.ball.vel * kBT2UU, ba = A.ball.ang;
+    o[0] = bp.x * s * px;
+    o[1] = bp.y * s * py;
+    o[2] = bp.z * pz;
+    o[3] = bv.x * s * vc;
+    o[4] = bv.y * s * vc;
+    o[5] = bv.z * vc;
+    o[6] = ba.x * s * ac;
+    o[7] = ba.y * s * ac;
+    o[8] = ba.z * ac;
+    for (int k = 0; k < kActionFloats; k++) o[9 + k] = prev[self][k];
+    for (int i = 0; i < kNumPads; i++) {
+        int slot = inv ? E.pad_map[kNumPads - 1 - i] : E.pad_map[i];
+        o[9 + kActionFloats + i] = A.pad[slot].is_active ? 1.f : 0.f;
+    }
+    int w = kObsHead;
+    obs_player<MAXC>(me, inv, o + w);
+    w += kPlayerObs;
+    for (int pass = 0; pass < 2; pass++) {  // teammates, then opponents, in slot order
+        for (int j = 0; j < A.ncars; j++) {
+            if (j == self) continue;
+            bool mate = A.car[j].team == me.team;
+            if (mate != (pass == 0)) continue;
+            obs_player<MAXC>(A.car[j], inv, o + w);
+            w += kPlayerObs;
+        }
+    }
+}
+
+RSX_HD bool touched_in_step(const Car& c, uint64_t tick, int tick_skip) {
+    return c.ball_hit_valid && c.ball_hit_tick >= tick - (uint64_t)tick_skip;
+}
+
+// Weighted sum of the four stub rewards for car i (reward-list order: goal,
+// touch, velocity ball->goal, velocity player->ball).
+template <int MAXC>
+RSX_HD float stub_reward(const Arena<MAXC>& A, int i, bool goal, bool touched, const EnvConfig& E) {
+    const Car& c = A.car[i];
+    V3 bp = A.ball.xf.o * kBT2UU, bv = A.ball.vel * kBT2UU;
+    V3 cp = c.xf.o * kBT2UU, cv = c.vel * kBT2UU;
+    float r = 0.f;
+    if (E.goal_w != 0.f) {
+        float g = 0.f;
+        if (goal) {
+            int ball_side = bp.y < 0 ? 0 : 1;  // RS_TEAM_FROM_Y
+            g = (c.team != ball_side) ? 1.f : E.concede_scale;
+        }
+        r += g * E.goal_w;
+    }
+    if (E.touch_w != 0.f) r += (touched ? 1.f : 0.f) * E.touch_w;
+    if (E.vbtg_w != 0.f) {
+        V3 target = c.team == 0 ? V3{0, kBackNetY, kGoalHeightRL / 2} : V3{0, -kBackNetY, kGoalHeightRL / 2};
+        V3 dir = rl_normalized(target - bp);
+        V3 nv = V3{bv.x / kRLBallMaxSpeed, bv.y / kRLBallMaxSpeed, bv.z / kRLBallMaxSpeed};
+        r += rl_dot(dir, nv) * E.vbtg_w;
+    }
+    if (E.vptb_w != 0.f) {
+        V3 dir = rl_normalized(bp - cp);
+        V3 nv = V3{cv.x / kRLCarMaxSpeed, cv.y / kRLCarMaxSpeed, cv.z / kRLCarMaxSpeed};
+        r += rl_dot(dir, nv) * E.vptb_w;
+    }
+    return r;
+}
+
+// Terminal type after a step of `step_dt` seconds; updates the no-touch timer.
+RSX_HD uint8_t step_terminal(bool goal, bool any_touch, float step_dt, float& no_touch, const EnvConfig& E) {
+    uint8_t t = kNotTerminal;
+    if (E.goal_terminal && goal) t = kTerminalNormal;
+    if (E.no_touch_timeout > 0.f) {
+        if (any_touch) {
+            no_touch = 0.f;
+        } else {
+            no_touch += step_dt;
+            if (no_touch >= E.no_touch_timeout && t == kNotTerminal) t = kTerminalTruncated;
+        }
+    }
+    return t;
+}
+
+}  // namespace rsx

@@ -442,6 +442,34 @@ rsx_controls controls_to_public(const Controls& c) {
     return r;
 }
 
+// RLGym's observation pad order (CommonValues.h:45-80 BOOST_LOCATIONS) mapped
+// onto the engine's pad indices by 2D position, as GameState's
+// boostPadIndexMap does (GameState.cpp:10-50).  Returns false if a location
+// has no unique match.
+bool obs_pad_map(const PadGeom* pads, int8_t* out) {
+    static const float loc[kNumPads][2] = {
+        {0.f, -4240.f}, {-1792.f, -4184.f}, {1792.f, -4184.f}, {-3072.f, -4096.f}, {3072.f, -4096.f},
+        {-940.f, -3308.f}, {940.f, -3308.f}, {0.f, -2816.f}, {-3584.f, -2484.f}, {3584.f, -2484.f},
+        {-1788.f, -2300.f}, {1788.f, -2300.f}, {-2048.f, -1036.f}, {0.f, -1024.f}, {2048.f, -1036.f},
+        {-3584.f, 0.f}, {-1024.f, 0.f}, {1024.f, 0.f}, {3584.f, 0.f}, {-2048.f, 1036.f},
+        {0.f, 1024.f}, {2048.f, 1036.f}, {-1788.f, 2300.f}, {1788.f, 2300.f}, {-3584.f, 2484.f},
+        {3584.f, 2484.f}, {0.f, 2816.f}, {-940.f, 3310.f}, {940.f, 3308.f}, {-3072.f, 4096.f},
+        {3072.f, 4096.f}, {-1792.f, 4184.f}, {1792.f, 4184.f}, {0.f, 4240.f}};
+    for (int i = 0; i < kNumPads; i++) {
+        int found = -1;
+        for (int j = 0; j < kNumPads; j++) {
+            float dx = pads[j].pos_uu.x - loc[i][0], dy = pads[j].pos_uu.y - loc[i][1];
+            if (dx * dx + dy * dy < 10.f) {
+                if (found >= 0) return false;
+                found = j;
+            }
+        }
+        if (found < 0) return false;
+        out[i] = (int8_t)found;
+    }
+    return true;
+}
+
 // RLGym's DefaultAction table (DefaultAction.cpp:3-59): 24 ground actions
 // (throttle x steer x boost x handbrake, boosting only at full throttle; yaw
 // follows steer), then 66 air actions (pitch x yaw x roll x jump x boost, no

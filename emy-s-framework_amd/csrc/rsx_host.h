@@ -47,6 +47,9 @@ void ball_to_public(const Ball& b, rsx_ball_state& s);
 Controls controls_from_public(const rsx_controls& c);
 rsx_controls controls_to_public(const Controls& c);
 
+// Observation pad slot -> engine pad index (RLGym BOOST_LOCATIONS order).
+bool obs_pad_map(const PadGeom* pads, int8_t* out34);
+
 // RLGym DefaultAction's 90-entry discrete action table, in its order.
 std::vector<rsx_controls> default_action_lut();
 

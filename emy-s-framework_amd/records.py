@@ -74,6 +74,12 @@ class Config(C.Structure):
                 ("seed", C.c_uint32), ("device", C.c_int32), ("car_order", C.c_int32)]
 
 
+class EnvConfig(C.Structure):
+    _fields_ = [("goal_weight", C.c_float), ("concede_scale", C.c_float), ("touch_weight", C.c_float),
+                ("vel_ball_to_goal_weight", C.c_float), ("vel_player_to_ball_weight", C.c_float),
+                ("no_touch_timeout_s", C.c_float), ("goal_terminal", C.c_int32), ("auto_reset", C.c_int32)]
+
+
 CARS_BLUE_FIRST = 0
 CARS_ALTERNATING = 1
 

@@ -116,6 +116,56 @@ rsx_status rsx_set_actions_device(rsx_engine* e, const int32_t* device_indices, 
 /* The default table (RLGym DefaultAction, DefaultAction.cpp:3-89), 90 entries. */
 rsx_status rsx_default_action_lut(rsx_controls* out, int32_t capacity, int32_t* count);
 
+/* Env half of a step (the RLGym DefaultObs + stub rewards + terminals the
+ * reference builds with BuildAdvancedObsAndDefaultMasks /
+ * BuildRewardsAndTerminals, RocketSimCuda.h:316, 338).  Rewards are the
+ * weighted sum, in this order, of GoalReward(concede_scale), TouchBallReward,
+ * VelocityBallToGoalReward and VelocityPlayerToBallReward
+ * (RLGymCPP RewardCore/CommonRewards.h:33-99).  Terminals per arena:
+ * 0 none, 1 normal (GoalScoreCondition), 2 truncated (NoTouchCondition),
+ * normal taking priority (EnvSet.cpp:178-200). */
+typedef struct rsx_env_config {
+    float goal_weight;                 /* default 1 */
+    float concede_scale;               /* default -1 */
+    float touch_weight;                /* default 1 */
+    float vel_ball_to_goal_weight;     /* default 1 */
+    float vel_player_to_ball_weight;   /* default 1 */
+    float no_touch_timeout_s;          /* NoTouchCondition(maxTime); <= 0 off; default rsx_config's */
+    int32_t goal_terminal;             /* GoalScoreCondition on; default 1 */
+    int32_t auto_reset;                /* kickoff reset of terminal arenas inside rsx_step_env; default 1 */
+} rsx_env_config;
+
+/* ConfigureTrainingRewards + ConfigureTrainingTerminals (RocketSimCuda.h:329-330). */
+rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c);
+rsx_status rsx_get_env_config(const rsx_engine* e, rsx_env_config* c);
+
+/* One fused env step for every arena, asynchronous on `stream`: device int32
+ * action indices (rsx_num_cars, clamped into the action table) -> controls,
+ * tick_skip physics ticks, rewards and terminals of the step, kickoff reset of
+ * terminal arenas (when auto_reset), then each car's DefaultObs row with the
+ * step's action as its previous action (zeros after a reset).  Replaces the
+ * caller sequence SetDiscreteActionsFromDevice + Step(tickSkip) +
+ * BuildAdvancedObsAndDefaultMasks + BuildRewardsAndTerminals +
+ * ResetTerminalArenasGpu (CudaEnvSet.cpp:364-376, 182-200). */
+rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stream);
+
+/* DefaultObs rows of the current state with zero previous actions (the first
+ * observation after create or a reset). */
+rsx_status rsx_build_obs(rsx_engine* e, void* stream);
+
+/* Device pointers of the env outputs (GetBuilt*DevicePtr, RocketSimCuda.h:
+ * 393-396): obs (obs_floats floats; car c's row at row_offset[c], length
+ * row_len[c] = 51 + 19 * cars in its arena: 89 / 127 / 165), rewards
+ * (rsx_num_cars floats), terminals (N bytes).  Stable for the engine's life. */
+rsx_status rsx_env_buffers(rsx_engine* e, float** obs, float** rewards, uint8_t** terminals, int64_t* obs_floats);
+rsx_status rsx_obs_layout(const rsx_engine* e, int64_t* row_offset, int32_t* row_len, int32_t count);
+
+/* Synchronous copies to host; replace CopyBuiltAdvancedObs / CopyBuiltRewards /
+ * CopyBuiltTerminals (RocketSimCuda.h:341-344). */
+rsx_status rsx_copy_obs(rsx_engine* e, float* out, int64_t count);
+rsx_status rsx_copy_rewards(rsx_engine* e, float* out, int32_t count);
+rsx_status rsx_copy_terminals(rsx_engine* e, uint8_t* out, int32_t count);
+
 /* Step every arena by `ticks` physics ticks; replaces Step
  * (RocketSimCuda.h:310, RocketSimCudaApi.cuh:551-571).  Asynchronous on
  * `stream` (a hipStream_t, NULL = default stream). */

@@ -4,6 +4,7 @@
 // Nothing in the engine's product path loads this library.
 #include "../../emy-s-framework_amd/csrc/rsx_host.h"
 #include "../../emy-s-framework_amd/csrc/rsx_soa.h"
+#include "../../emy-s-framework_amd/csrc/rsx_env.h"
 
 #include <cstring>
 #include <string>
@@ -207,4 +208,39 @@ extern "C" void cc_raytest_static(void* w, const float* a, const float* b, float
     out[0] = hit.object >= 0;
     out[1] = hit.fraction;
     out[5] = hit.normal.x; out[6] = hit.normal.y; out[7] = hit.normal.z;
+}
+
+// Env outputs of the current state after a step of tick_skip ticks, with the
+// default stub-reward weights (1, concede -1, 1, 1, 1): each car's DefaultObs
+// row into obs (rows back to back), rewards, and the terminal type (return
+// value) with the no-touch timer carried through *no_touch.
+extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, float* no_touch, float* obs,
+                      float* rew) {
+    auto* a = (CcArena*)p;
+    EnvConfig E;
+    E.goal_w = 1.f;
+    E.concede_scale = -1.f;
+    E.touch_w = 1.f;
+    E.vbtg_w = 1.f;
+    E.vptb_w = 1.f;
+    E.no_touch_timeout = timeout;
+    E.goal_terminal = 1;
+    E.auto_reset = 0;
+    if (!obs_pad_map(a->world->w.pads, E.pad_map)) return -1;
+    const Arena<6>& A = a->A;
+    float pa[6][kActionFloats];
+    for (int i = 0; i < A.ncars; i++)
+        for (int k = 0; k < kActionFloats; k++) pa[i][k] = prev[i * kActionFloats + k];
+    bool goal = ball_scored_now(A.ball);
+    bool any = false;
+    bool touched[6];
+    for (int i = 0; i < A.ncars; i++) {
+        touched[i] = touched_in_step(A.car[i], A.tick, tick_skip);
+        any |= touched[i];
+    }
+    uint8_t t = step_terminal(goal, any, (float)tick_skip * (1.f / kTickRate), *no_touch, E);
+    for (int i = 0; i < A.ncars; i++) rew[i] = stub_reward<6>(A, i, goal, touched[i], E);
+    int row = obs_row_len(A.ncars);
+    for (int i = 0; i < A.ncars; i++) obs_row<6>(A, i, pa, E, obs + i * row);
+    return t;
 }

@@ -93,6 +93,7 @@ def load_oracle():
         ("ref_get_aabb", None, [VP, C.c_int, C.POINTER(C.c_float)]),
         ("ref_pad_config", None, [VP, C.POINTER(C.c_float)]),
         ("ref_sizeof_car_state", C.c_uint32, []),
+        ("ref_num_cars", C.c_int, [VP]),
     ])
     if lib is not None:
         assert lib.ref_init(MESH_DIR.encode()) == 0
@@ -119,7 +120,21 @@ def load_cpucheck():
         ("cc_get_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
         ("cc_step", C.c_int, [VP, C.c_int]),
         ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+        ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
+                             C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ])
+
+
+def load_spec():
+    """The CPU restatement of the RLGym obs / reward / terminal pieces (oracle/rlgym_spec.py)."""
+    if "rlgym_spec" in sys.modules:
+        return sys.modules["rlgym_spec"]
+    path = os.path.join(ROOT, "oracle", "rlgym_spec.py")
+    spec = importlib.util.spec_from_file_location("rlgym_spec", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["rlgym_spec"] = mod
+    spec.loader.exec_module(mod)
+    return mod
 
 
 def parse_contacts(buf, n):

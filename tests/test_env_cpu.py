@@ -1,0 +1,103 @@
+"""Env outputs (DefaultObs rows, stub rewards, terminals) of the engine's own
+env code compiled for the host (tests/cpu_check cc_env) against the CPU
+restatement (oracle/rlgym_spec.py): on the engine's states to float rounding,
+and on the oracle's states run in lockstep to the physics tolerance carried
+through the obs scaling."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import parity_harness as P
+import rsxlib as R
+
+S = R.load_spec()
+W = dict(goal=1, concede=-1, touch=1, vbtg=1, vptb=1)
+
+
+def _lut():
+    return [(c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump, c.boost, c.handbrake)
+            for c in R.engine.default_action_lut()]
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_env_outputs_match_spec(seed, oracle, cpucheck):
+    if not __import__("os").path.exists(R.engine.LIB_PATH):
+        pytest.skip("librsx.so is not built")
+    lib, world = cpucheck
+    lut = _lut()
+    rng = np.random.default_rng(seed)
+    cars = [P.default_car(pos=(-300, -900, 17), yaw=1.2, vel=(100, 800, 0), boost=60),
+            P.default_car(pos=(500, 1200, 17), yaw=-2.0, vel=(-300, -900, 0), team=1, boost=20)]
+    ball = P.ball((-150, -520, 93.15))  # in the blue car's path
+    case = dict(blue=1, orange=1, cars=cars, ball=ball, ticks=0, controls=[P.controls(), P.controls()])
+    # lockstep: cpucheck engine and oracle, same states, 8-tick steps
+    h = C.c_void_p(lib.cc_arena_new(world, 1, 1))
+    o = C.c_void_p(oracle.ref_arena_new(1, 1))
+    oracle.ref_step(o, 1)
+    tick0 = 1000
+    lib.cc_set_tick(h, tick0)
+    oracle.ref_set_tick(o, tick0)
+    pads = R.PADS()
+    for i in range(34):
+        pads[i].is_active = 1
+    for i, s in enumerate(cars):
+        lib.cc_set_car(h, i, C.byref(s))
+        lib.cc_set_wheels(h, i, P.WHEEL_ZERO6)
+        oracle.ref_set_car(o, i, C.byref(s))
+        oracle.ref_set_wheels(o, i, P.WHEEL_ZERO)
+    lib.cc_set_ball(h, C.byref(ball))
+    oracle.ref_set_ball(o, C.byref(ball))
+    lib.cc_set_pads(h, pads)
+    oracle.ref_set_pads(o, pads)
+    nt_eng = C.c_float(0)
+    nt_spec_e = nt_spec_o = 0.0
+    obs = (C.c_float * (2 * 89))()
+    rew = (C.c_float * 2)()
+    try:
+        seen = set()
+        for step in range(24):
+            idx = rng.integers(0, 90, size=2) if step > 3 else np.array([21, 20])  # drive at first
+            prev = np.array([lut[k] for k in idx], dtype=np.float32)
+            for i in range(2):
+                ctl = R.Controls()
+                ctl.throttle, ctl.steer, ctl.pitch, ctl.yaw, ctl.roll, ctl.jump, ctl.boost, ctl.handbrake = \
+                    lut[idx[i]]
+                lib.cc_set_controls(h, i, C.byref(ctl))
+                oracle.ref_set_controls(o, i, C.byref(ctl))
+            lib.cc_step(h, 8)
+            oracle.ref_step(o, 8)
+            pv = (C.c_float * 16)(*prev.ravel())
+            term = lib.cc_env(h, pv, 8, 1.0, C.byref(nt_eng), obs, rew)
+            ecars = [R.CarState() for _ in range(2)]
+            ocars = [R.CarState() for _ in range(2)]
+            for i in range(2):
+                lib.cc_get_car(h, i, C.byref(ecars[i]))
+                oracle.ref_get_car(o, i, C.byref(ocars[i]))
+            eb, ob = R.BallState(), R.BallState()
+            lib.cc_get_ball(h, C.byref(eb))
+            oracle.ref_get_ball(o, C.byref(ob))
+            ep, op = R.PADS(), R.PADS()
+            lib.cc_get_pads(h, ep)
+            oracle.ref_get_pads(o, op)
+            tick = tick0 + 8 * (step + 1)
+            got = np.array(obs, dtype=np.float32).reshape(2, 89)
+            for states, cars_, ball_, pads_, tol in (("engine", ecars, eb, ep, 2e-6), ("oracle", ocars, ob, op, 2e-4)):
+                touched = [S.touched_in_step(c, tick, 8) for c in cars_]
+                goal = S.ball_scored(ball_)
+                want_obs = np.stack([S.default_obs(cars_, ball_, pads_, prev, i) for i in range(2)])
+                np.testing.assert_allclose(got, want_obs, atol=tol, err_msg=f"obs vs spec on {states} states, step {step}")
+                want_rew = S.stub_rewards(cars_, ball_, goal, touched, W)
+                np.testing.assert_allclose(np.array(rew), want_rew, atol=tol * 2, err_msg=f"rewards, {states}, step {step}")
+                if states == "engine":
+                    t, nt_spec_e = S.step_terminal(goal, any(touched), 8 / 120, nt_spec_e, 1.0)
+                else:
+                    t, nt_spec_o = S.step_terminal(goal, any(touched), 8 / 120, nt_spec_o, 1.0)
+                assert term == t, (states, step)
+            seen.add(("touch", any(S.touched_in_step(c, tick, 8) for c in ecars)))
+            seen.add(("term", term))
+        # the run covers a touch and a no-touch truncation
+        assert ("touch", True) in seen and ("term", 2) in seen, seen
+    finally:
+        lib.cc_arena_free(h)
+        oracle.ref_arena_free(o)
