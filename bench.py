@@ -4,8 +4,10 @@ Workload (default): 65536 soccar 1v1 arenas from the random-state workload
 (emy-s-framework_amd/workloads.py, seed 4), a fresh uniform discrete action
 per car each step (device int32 indices over the 90-entry default table),
 tick-skip 8 at 120 Hz, exact collision.  One step = every arena advances one
-env step (8 physics ticks).  Inputs are resident in HBM when the timed region
-starts.
+env step: 8 physics ticks, the four stub rewards, goal / no-touch terminals,
+kickoff auto-reset of terminal arenas and each car's DefaultObs row, all in
+one fused launch (rsx_step_env).  Inputs are resident in HBM when the timed
+region starts.
 
 Prints one JSON line: throughput in env-steps/s (arenas x steps / s), the
 physics kernel's HBM roofline fraction measured with HIP events on the stream
@@ -23,10 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "emy-s-framework_amd")
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of one 1v1 env step of the physics path: arena state read
-# once and written once (2 x 1132 B: two cars of 400 B, ball 36 B, 34 pads of
-# 8 B, arena record 24 B) plus one int32 action per car (2 x 4 B).
-ALGO_BYTES_1V1 = 2 * 1132 + 2 * 4
+# Algorithmic bytes of one 1v1 env step: arena state read once and written
+# once (2 x 1132 B: two cars of 400 B, ball 36 B, 34 pads of 8 B, arena record
+# 24 B), one int32 action per car in (2 x 4 B), per car one 89-float DefaultObs
+# row and one float reward out (2 x 356 B + 2 x 4 B), one terminal byte.
+ALGO_BYTES_1V1 = 2 * 1132 + 2 * 4 + 2 * 89 * 4 + 2 * 4 + 1
 
 
 def _import_package():
@@ -125,6 +128,7 @@ def main():
     dev = torch.device("cuda:0")
     batch = eng.RocketSimBatch(args.arenas, 1, tick_skip=args.tick_skip, seed=4, device=0)
     random_field_states(batch, seed=4)
+    batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
     n_cars = batch.num_cars
     stream = torch.cuda.current_stream(dev)
     sp = C.c_void_p(stream.cuda_stream)
@@ -136,10 +140,9 @@ def main():
 
     def step(i, ev=None):
         a = actions[i % len(actions)]
-        batch.SetDiscreteActionsFromDevice(a.data_ptr(), n_cars, sp)
         if ev is not None:
             ev[0].record(stream)
-        batch.Step(args.tick_skip, sp)
+        batch.StepEnv(a.data_ptr(), sp)
         if ev is not None:
             ev[1].record(stream)
 
@@ -176,12 +179,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{args.arenas} soccar 1v1 arenas, random field states (seed 4), uniform random "
-                               f"discrete actions, tick-skip {args.tick_skip}, exact collision, physics step only "
-                               f"(no obs/rewards yet)",
+                               f"discrete actions, tick-skip {args.tick_skip}, exact collision, fused env step "
+                               f"(physics + DefaultObs + 4 stub rewards + terminals + auto-reset)",
                    "arenas": args.arenas, "tick_skip": args.tick_skip, "cull": "none (exact collision)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step<2>", "kernel_ms": kern_ms, "algo_bytes_per_env_step": ALGO_BYTES_1V1},
+                     "kernel": "k_env_step<2>", "kernel_ms": kern_ms, "algo_bytes_per_env_step": ALGO_BYTES_1V1},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

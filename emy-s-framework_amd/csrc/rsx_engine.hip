@@ -58,6 +58,7 @@ struct Bucket {
     rsx_controls* d_ctrl = nullptr;  // n * maxc
     int* d_car_off = nullptr;        // n: global index of each arena's first car
     int64_t* d_obs_off = nullptr;    // n: float offset of each arena's first obs row
+    int* d_global = nullptr;         // n: global arena index
     std::vector<int> global;         // bucket-local -> global arena index
 };
 
@@ -152,7 +153,8 @@ template <int MAXC>
 __global__ void __launch_bounds__(64) k_env_step(uint32_t* words, int n, const int32_t* act, const int* car_off,
                                                  const rsx_controls* lut, int lut_n, int ticks, CarGeom cg,
                                                  BallGeom bg, MeshView mv, const PadGeom* pads, EnvConfig E,
-                                                 float* obs, const int64_t* obs_off, float* rew, uint8_t* term) {
+                                                 float* obs, const int64_t* obs_off, float* rew, uint8_t* term,
+                                                 const int* global) {
     using L = Layout<MAXC>;
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
@@ -215,7 +217,7 @@ __global__ void __launch_bounds__(64) k_env_step(uint32_t* words, int n, const i
     col.set(L::kArenaBase + kArFlags, flags);
     col.setf(L::kArenaBase + kArNoTouch, no_touch);
     col.set(L::kArenaBase + kArEpisode, episode);
-    term[a] = tt;
+    term[global[a]] = tt;
 }
 
 // DefaultObs rows of the current state with zero previous actions (the
@@ -432,6 +434,8 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
             if (st != RSX_OK) return st;
             st = upload(&b.d_obs_off, ooff);
             if (st != RSX_OK) return st;
+            st = upload(&b.d_global, b.global);
+            if (st != RSX_OK) return st;
         }
         HIP_TRY(hipMalloc((void**)&b.d_words, sizeof(uint32_t) * (size_t)b.words * b.n));
         HIP_TRY(hipMalloc((void**)&b.d_ctrl, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
@@ -459,6 +463,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
         (void)hipFree(b.d_ctrl);
         (void)hipFree(b.d_car_off);
         (void)hipFree(b.d_obs_off);
+        (void)hipFree(b.d_global);
     }
     (void)hipFree(e->d_obs);
     (void)hipFree(e->d_rew);
@@ -899,7 +904,7 @@ rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stre
 #define RSX_LAUNCH_ENV(M)                                                                                        \
     hipLaunchKernelGGL(k_env_step<M>, grid, block, 0, st, b.d_words, b.n, device_actions, b.d_car_off, e->d_lut, \
                        e->lut_n, e->tick_skip, e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->env, e->d_obs,       \
-                       b.d_obs_off, e->d_rew, e->d_term + 0)
+                       b.d_obs_off, e->d_rew, e->d_term, b.d_global)
         if (b.maxc == 2) RSX_LAUNCH_ENV(2);
         else if (b.maxc == 4) RSX_LAUNCH_ENV(4);
         else RSX_LAUNCH_ENV(6);

@@ -24,9 +24,9 @@ def _lut():
 def _cases():
     a = [P.default_car(pos=(-300, -900, 17), yaw=1.2, vel=(100, 800, 0), boost=60),
          P.default_car(pos=(500, 1200, 17), yaw=-2.0, vel=(-300, -900, 0), team=1, boost=20)]
-    b = [P.default_car(pos=(2000, -3000, 17), yaw=0.3, boost=100),
+    b = [P.default_car(pos=(500, -1500, 17), yaw=0.3, boost=100),
          P.default_car(pos=(-2500, 3500, 17), yaw=2.8, vel=(0, -1500, 0), team=1)]
-    c = [P.default_car(pos=(0, 4500, 17), yaw=1.5708, vel=(0, 1400, 0)),
+    c = [P.default_car(pos=(-1000, 2500, 17), yaw=0.2, vel=(300, 0, 0)),
          P.default_car(pos=(-3000, -3000, 17), yaw=0.7, team=1)]
     return [(a, P.ball((-150, -520, 93.15))),
             (b, P.ball((1000, 1000, 800), (-500, 300, -200), (1, 0, 2))),
@@ -133,7 +133,7 @@ def test_auto_reset_after_goal():
     rew = b.CopyBuiltRewards()
     assert list(term) == [0, 1, 0, 0]
     info = b.GetArenaInfo(1)
-    assert info.tick_count == 0 and info.episode == 2 and info.goal_scored == 0
+    assert info.tick_count == 0 and info.episode == 1 and info.goal_scored == 0
     ball = b.GetBallState(1)
     assert ball.pos.x == 0 and ball.pos.y == 0 and abs(ball.pos.z - 93.15) < 1e-3
     # arena 1 is 2v2 (cars 2..5): blue +1, orange -1 (goal term of the reward sum)
