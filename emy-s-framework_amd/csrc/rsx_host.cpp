@@ -166,6 +166,123 @@ static void pad_geometry(PadGeom* pads) {
 
 static void internal_edge_info(HostWorld& w);
 
+// Leaf order of the quantized BVH that btBvhTriangleMeshShape builds over one
+// mesh (btOptimizedBvh::build, btQuantizedBvh::setQuantizationValues /
+// quantize / unQuantize / calcSplittingAxis / sortAndCalcSplittingIndex /
+// buildTree, bullet3-3.24 BroadphaseCollision/btQuantizedBvh.cpp).  The
+// stackless traversal visits overlapping leaves in node-array order, which is
+// the final order of the leaf array after the in-place partitions; the
+// returned rank[t] is triangle t's position in it.
+namespace {
+struct QBvh {
+    V3 amin, amax, quant;
+    struct Leaf {
+        uint16_t qmin[3], qmax[3];
+        int tri;
+    };
+    std::vector<Leaf> leaves;
+
+    void quantize(uint16_t* out, V3 p, bool is_max) const {
+        V3 v = (p - amin) * quant;
+        if (is_max) {
+            out[0] = (uint16_t)(((uint16_t)(v.x + 1.f)) | 1);
+            out[1] = (uint16_t)(((uint16_t)(v.y + 1.f)) | 1);
+            out[2] = (uint16_t)(((uint16_t)(v.z + 1.f)) | 1);
+        } else {
+            out[0] = (uint16_t)(((uint16_t)(v.x)) & 0xfffe);
+            out[1] = (uint16_t)(((uint16_t)(v.y)) & 0xfffe);
+            out[2] = (uint16_t)(((uint16_t)(v.z)) & 0xfffe);
+        }
+    }
+    V3 unquantize(const uint16_t* q) const {
+        V3 o = V3{(float)q[0] / quant.x, (float)q[1] / quant.y, (float)q[2] / quant.z};
+        return o + amin;
+    }
+    void set_quantization(V3 mn, V3 mx) {
+        const V3 clamp = V3{1.f, 1.f, 1.f};
+        const V3 q65533 = V3{65533.f, 65533.f, 65533.f};
+        amin = mn - clamp;
+        amax = mx + clamp;
+        V3 size = amax - amin;
+        quant = V3{q65533.x / size.x, q65533.y / size.y, q65533.z / size.z};
+        uint16_t vin[3];
+        quantize(vin, amin, false);
+        V3 v = unquantize(vin);
+        amin = vmin(amin, v - clamp);
+        size = amax - amin;
+        quant = V3{q65533.x / size.x, q65533.y / size.y, q65533.z / size.z};
+        quantize(vin, amax, true);
+        v = unquantize(vin);
+        amax = vmax(amax, v + clamp);
+        size = amax - amin;
+        quant = V3{q65533.x / size.x, q65533.y / size.y, q65533.z / size.z};
+    }
+    V3 center(int i) const { return 0.5f * (unquantize(leaves[i].qmax) + unquantize(leaves[i].qmin)); }
+    int split_axis(int s, int e) const {
+        V3 means = V3{0, 0, 0}, var = V3{0, 0, 0};
+        int n = e - s;
+        for (int i = s; i < e; i++) means += center(i);
+        means *= (1.f / (float)n);
+        for (int i = s; i < e; i++) {
+            V3 d = center(i) - means;
+            var += d * d;
+        }
+        var *= (1.f / ((float)n - 1));
+        return gjk::max_axis(var);
+    }
+    int split_index(int s, int e, int axis) {
+        int split = s, n = e - s;
+        V3 means = V3{0, 0, 0};
+        for (int i = s; i < e; i++) means += center(i);
+        means *= (1.f / (float)n);
+        float sv = idx(means, axis);
+        for (int i = s; i < e; i++) {
+            if (idx(center(i), axis) > sv) {
+                std::swap(leaves[i], leaves[split]);
+                split++;
+            }
+        }
+        int bal = n / 3;
+        bool unbalanced = (split <= (s + bal)) || (split >= (e - 1 - bal));
+        if (unbalanced) split = s + (n >> 1);
+        return split;
+    }
+    void build(int s, int e) {
+        if (e - s == 1) return;
+        int axis = split_axis(s, e);
+        int split = split_index(s, e, axis);
+        build(s, split);
+        build(split, e);
+    }
+};
+}  // namespace
+
+static std::vector<int> bvh_leaf_rank(const float* tri, int nt, V3 mn, V3 mx) {
+    QBvh q;
+    q.set_quantization(mn, mx);
+    q.leaves.resize(nt);
+    for (int t = 0; t < nt; t++) {
+        const float* p = tri + (size_t)t * 9;
+        V3 lo = V3{kLargeFloat, kLargeFloat, kLargeFloat}, hi = -lo;
+        for (int k = 0; k < 3; k++) {
+            V3 v = V3{p[k * 3], p[k * 3 + 1], p[k * 3 + 2]};
+            lo = vmin(lo, v);
+            hi = vmax(hi, v);
+        }
+        const float kMinDim = 0.002f, kMinHalf = 0.001f;
+        if (hi.x - lo.x < kMinDim) { hi.x += kMinHalf; lo.x -= kMinHalf; }
+        if (hi.y - lo.y < kMinDim) { hi.y += kMinHalf; lo.y -= kMinHalf; }
+        if (hi.z - lo.z < kMinDim) { hi.z += kMinHalf; lo.z -= kMinHalf; }
+        q.quantize(q.leaves[t].qmin, lo, false);
+        q.quantize(q.leaves[t].qmax, hi, true);
+        q.leaves[t].tri = t;
+    }
+    q.build(0, nt);
+    std::vector<int> rank(nt);
+    for (int i = 0; i < nt; i++) rank[q.leaves[i].tri] = i;
+    return rank;
+}
+
 bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, std::string& err) {
     if (files.size() != (size_t)kNumMeshes) {
         err = "expected 16 collision mesh files";
@@ -211,6 +328,12 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
             }
             w.tri_mesh.push_back(m);
             w.tri_rank.push_back(t);
+        }
+        {
+            const size_t base = w.tri.size() - (size_t)nt * 9;
+            std::vector<int> rank = bvh_leaf_rank(&w.tri[base], nt, mn, mx);
+            const size_t tbase = w.tri_rank.size() - (size_t)nt;
+            for (int t = 0; t < nt; t++) w.tri_rank[tbase + t] = rank[t];
         }
         // btTriangleMeshShape::getAabb at identity: centre +- half extents
         V3 half = (mx - mn) * 0.5f;

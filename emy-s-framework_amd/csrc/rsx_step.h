@@ -5,6 +5,7 @@
 // boost pads, ball limits and the goal test.
 #pragma once
 
+#include "rsx_gjk.h"
 #include "rsx_solver.h"
 
 namespace rsx {
@@ -54,6 +55,7 @@ struct TickScratch {
     Solver<MAXC> S;
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
     int cand[48];
+    gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
 };
 
 template <int MAXC>
@@ -140,6 +142,49 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
         }
     }
     manifold_refresh(m, b.xf, trB);
+}
+
+// Narrowphase of car `ci`'s box against mesh `mesh`
+// (btConvexConcaveCollisionAlgorithm with the box as the convex body: the
+// triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
+// one manifold per car and mesh, refreshed after all triangles).
+template <int MAXC>
+RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifold& m, int ci, int mesh,
+                        const int* cand, int ncand) {
+    const CarGeom& g = T.cg;
+    Car& c = A.car[ci];
+    const Xf child = car_child_xf(g, c.xf);
+    const Xf trB = xf_identity();
+    const M3 inv_basis = inverse(child.b);
+    const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+    const float max_d = g.box_margin + 0.f + m.breaking;
+    for (int i = 0; i < ncand; i++) {
+        int t = cand[i];
+        if (T.mv.tri_mesh[t] != mesh) continue;
+        gjk::Shapes sh;
+        load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
+        // early out: the box is wholly on one side of the triangle's plane
+        {
+            V3 n = normalized(cross(sh.tri[1] - sh.tri[0], sh.tri[2] - sh.tri[0]));
+            bool skip = false;
+            for (int side = 0; side < 2 && !skip; side++) {
+                V3 lp = gjk::box_support(hm, inv_basis * n);
+                V3 wp = xform(child, lp);
+                float dist = dot(n, sh.tri[0]) - dot(n, wp);
+                if (dist > m.breaking) skip = true;
+                n = n * -1.f;
+            }
+            if (skip) continue;
+        }
+        sh.h = g.box_implicit;
+        sh.margin = g.box_margin;
+        gjk::Contact1 ct;
+        if (gjk::closest_points(G, sh, child, trB, max_d * max_d, ct)) {
+            int idx = manifold_add(m, c.xf, trB, ct.normal_on_b, ct.point_on_b, ct.depth, 0.6f, 0.3f);
+            if (idx >= 0) contact_added(A, T, m, idx, t);
+        }
+    }
+    if (m.n) manifold_refresh(m, c.xf, trB);
 }
 
 // GJK result for the ball's centre point against the car's core box (margins
@@ -258,8 +303,15 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                         ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
                     }
                     ball_vs_mesh(A, T, m, ms, W.cand, ncand);
+                } else {
+                    if (ncand < 0) {
+                        Xf child = car_child_xf(g, A.car[d - 1].xf);
+                        V3 cmn, cmx;
+                        box_child_aabb(g, child, cmn, cmx);
+                        ncand = gather_tris(T.mv, cmn, cmx, W.cand, 48);
+                    }
+                    car_vs_mesh(A, T, W.gj, m, d - 1, ms, W.cand, ncand);
                 }
-                // car vs mesh triangles: not generated yet
             }
             mf_close(L, k);
         }
