@@ -19,6 +19,7 @@
 //                                  bullet3-3.24/BulletDynamics/ConstraintSolver/btContactConstraint.cpp
 #pragma once
 
+#include "rsx_gjk.h"
 #include "rsx_query.h"
 
 namespace rsx {
@@ -77,34 +78,61 @@ RSX_HD void wheel_update_transform(Car& c, const CarGeom& g, int i) {
     w.wbasis = steer * b2;
 }
 
-// Bullet ray test for a suspension ray: static world, then the other dynamic
-// bodies (btRSBroadphase::rayTest visits static handles before dynamic ones).
+// World transform of the car's compound child box (colObjWorldTransform *
+// childTrans; the child has an identity basis and the hitbox offset).
+RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
+    return Xf{car.b * m3_identity(), xform(car, g.box_offset)};
+}
+
+// Bullet ray test for a suspension ray (btDefaultVehicleRaycaster::castRay,
+// btDefaultVehicleRaycaster.cpp:32-53): static world first, then the dynamic
+// bodies (btRSBroadphase::rayTest visits a cell's static handles before its
+// dynamic ones, btRSBroadphase.cpp:326-337), the ray's own chassis skipped
+// (RayResultCallback::needsCollision, btCollisionWorld.cpp:1208-1214).  Convex
+// bodies are cast with btSubsimplexConvexCast (btCollisionWorld.cpp:267-312):
+// the ball's sphere, and each car's compound child box (the compound has no
+// child tree, Car.cpp:198, so its one child is tested directly).  A closest
+// body without contact response -- a car whose rigid body is disabled this
+// tick -- gives no hit at all.
+//
+// Every car whose wheels are cast is live, so its own AABB marks the
+// suspension grid cells around its hardpoints and the grid always defers to
+// the full ray test (SuspensionCollisionGrid.cpp:131-133); and a dynamic body
+// is registered in the 3x3x3 broadphase cells around its AABB minimum, which
+// covers every body a suspension-length ray from the origin's cell can reach.
 template <int MAXC>
-RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const BallGeom& bg, int self, V3 from, V3 to,
-                      RayHit& hit) {
+RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int self,
+                      V3 from, V3 to, RayHit& hit) {
     hit.object = -1;
     hit.fraction = 1.0f;
     ray_static_world(mv, from, to, hit);
-    // ball: ray against its collision sphere
+    auto try_body = [&](const gjk::RayShape& s, int object) {
+        float f;
+        V3 n;
+        if (!gjk::ray_cast(s, from, to, f, n)) return;
+        if (!(len2(n) > 0.0001f)) return;
+        float cur = hit.object >= 0 ? hit.fraction : 1.0f;
+        if (!(f < cur)) return;
+        hit.fraction = f;
+        hit.normal = normalized(n);
+        hit.object = object;
+    };
     {
-        V3 d = to - from;
-        V3 m = from - A.ball.xf.o;
-        float a = dot(d, d);
-        float b = dot(m, d);
-        float cc = dot(m, m) - bg.radius * bg.radius;
-        float disc = b * b - a * cc;
-        if (cc > 0.f && b < 0.f && disc >= 0.f) {
-            float t = (-b - sqrtf(disc)) / a;
-            float cur = hit.object >= 0 ? hit.fraction : 1.0f;
-            if (t >= 0.f && t < cur) {
-                hit.fraction = t;
-                V3 p = from + d * t;
-                hit.normal = normalized(p - A.ball.xf.o);
-                hit.object = 1;  // dynamic body 0 (ball)
-            }
-        }
+        gjk::RayShape s;
+        s.kind = 1;
+        s.h = V3{bg.radius, 0, 0};
+        s.xf = A.ball.xf;
+        try_body(s, 1);
     }
-    (void)self;
+    for (int k = 0; k < A.ncars; k++) {
+        if (k == self) continue;
+        gjk::RayShape s;
+        s.kind = 0;
+        s.h = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+        s.xf = car_child_xf(g, A.car[k].xf);
+        try_body(s, 2 + k);
+    }
+    if (hit.object >= 2 && !A.car[hit.object - 2].active) hit.object = -1;
 }
 
 // btVehicleRL::rayCast (btVehicleRL.cpp:118-212)
@@ -123,7 +151,7 @@ RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGe
     w.contact_point = target;
     w.ground = -1;
     RayHit hit;
-    ray_world(mv, A, bg, ci, source, target, hit);
+    ray_world(mv, A, g, bg, ci, source, target, hit);
     if (hit.object >= 0) {
         float s = 1.0f - hit.fraction;
         w.contact_point = V3{s * source.x + hit.fraction * target.x, s * source.y + hit.fraction * target.y,
@@ -179,8 +207,44 @@ RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGe
     }
 }
 
-// btVehicleRL::calcFrictionImpulses (btVehicleRL.cpp:313-387); static ground only
-RSX_HD void vehicle_friction_impulses(Car& c, const CarGeom& g, float dt, const Ball& ball, const BallGeom& bg) {
+// Ground body of a wheel contact as resolveSingleBilateral sees it
+// (btVehicleRL.cpp:321-343): velocity, inverse mass, local inverse inertia
+// diagonal and centre of mass transform; the static world is the fixed body.
+struct GroundBody {
+    V3 vel, ang, inv_inertia_local, com;
+    M3 basis;
+    float inv_mass;
+};
+
+template <int MAXC>
+RSX_HD GroundBody ground_body(const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int object) {
+    GroundBody b;
+    if (object == 1) {
+        b.vel = A.ball.vel;
+        b.ang = A.ball.ang;
+        b.inv_mass = bg.inv_mass;
+        b.inv_inertia_local = V3{bg.inv_inertia, bg.inv_inertia, bg.inv_inertia};
+        b.com = A.ball.xf.o;
+        b.basis = A.ball.xf.b;
+    } else if (object >= 2) {
+        const Car& k = A.car[object - 2];
+        b.vel = k.vel;
+        b.ang = k.ang;
+        b.inv_mass = g.inv_mass;
+        b.inv_inertia_local = g.inv_inertia_local;
+        b.com = k.xf.o;
+        b.basis = k.xf.b;
+    } else {
+        b.vel = b.ang = b.inv_inertia_local = b.com = V3{0, 0, 0};
+        b.basis = m3_identity();
+        b.inv_mass = 0.f;
+    }
+    return b;
+}
+
+// btVehicleRL::calcFrictionImpulses (btVehicleRL.cpp:313-387)
+template <int MAXC>
+RSX_HD void vehicle_friction_impulses(const Arena<MAXC>& A, Car& c, const CarGeom& g, float dt, const BallGeom& bg) {
     float friction_scale = car_mass(g) / 3;
     for (int i = 0; i < 4; i++) {
         Wheel& w = c.wheel[i];
@@ -192,28 +256,19 @@ RSX_HD void vehicle_friction_impulses(Car& c, const CarGeom& g, float dt, const 
             axle = safe_normalized(axle);
             V3 fwd = safe_normalized(cross(n, axle));
             // resolveSingleBilateral(chassis, p, ground, p, 0, axle, side, dt)
+            const GroundBody gb = ground_body(A, g, bg, w.ground);
             float side;
             {
                 V3 rel1 = w.contact_point - c.xf.o;
                 V3 v1 = car_vel_at(c, rel1);
-                V3 v2 = V3{0, 0, 0};
-                float inv_mass_b = 0.f;
-                V3 inv_inertia_b = V3{0, 0, 0};
-                M3 world2b = m3_identity();
-                V3 rel2 = w.contact_point;
-                if (w.ground == 1) {
-                    rel2 = w.contact_point - ball.xf.o;
-                    v2 = ball.vel + cross(ball.ang, rel2);
-                    inv_mass_b = bg.inv_mass;
-                    inv_inertia_b = V3{bg.inv_inertia, bg.inv_inertia, bg.inv_inertia};
-                    world2b = transpose(ball.xf.b);
-                }
+                V3 rel2 = w.contact_point - gb.com;
+                V3 v2 = gb.vel + cross(gb.ang, rel2);
                 V3 vel = v1 - v2;
                 V3 aJ = transpose(c.xf.b) * cross(rel1, axle);
-                V3 bJ = world2b * cross(rel2, -axle);
+                V3 bJ = transpose(gb.basis) * cross(rel2, -axle);
                 V3 m0 = g.inv_inertia_local * aJ;
-                V3 m1 = inv_inertia_b * bJ;
-                float adiag = g.inv_mass + dot(m0, aJ) + inv_mass_b + dot(m1, bJ);
+                V3 m1 = gb.inv_inertia_local * bJ;
+                float adiag = g.inv_mass + dot(m0, aJ) + gb.inv_mass + dot(m1, bJ);
                 float jac_inv = 1.f / adiag;
                 float rel_vel = dot(axle, vel);
                 side = -0.2f * rel_vel * jac_inv;
@@ -225,7 +280,7 @@ RSX_HD void vehicle_friction_impulses(Car& c, const CarGeom& g, float dt, const 
                     V3 car_rel = cp - c.xf.o;
                     V3 v1 = car_vel_at(c, car_rel);
                     V3 v2 = V3{0, 0, 0};
-                    if (w.ground == 1) v2 = ball.vel + cross(ball.ang, car_rel);
+                    if (w.ground >= 1) v2 = gb.vel + cross(gb.ang, car_rel);
                     V3 cv = v1 - v2;
                     float rv = dot(cv, fwd);
                     if (dt > (1 / 80.f)) {
@@ -590,7 +645,7 @@ RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeo
     // updateVehicleFirst
     for (int i = 0; i < 4; i++) wheel_update_transform(c, g, i);
     for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt);
-    vehicle_friction_impulses(c, g, dt, A.ball, bg);
+    vehicle_friction_impulses(A, c, g, dt, bg);
 
     bool jump_pressed = c.controls.jump && !c.last_controls.jump;
     int nwc = 0;

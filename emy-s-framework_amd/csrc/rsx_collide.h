@@ -196,9 +196,6 @@ RSX_HD void box_child_aabb(const CarGeom& g, const Xf& child, V3& mn, V3& mx) {
     mn = child.o - ext;
     mx = child.o + ext;
 }
-RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
-    return Xf{car.b * m3_identity(), xform(car, g.box_offset)};
-}
 
 RSX_HD void broad_aabb(V3 mn1, V3 mx1, V3 mn2, V3 mx2, V3& mn, V3& mx) {
     const float t = kContactBreakingThreshold;

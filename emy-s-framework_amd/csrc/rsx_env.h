@@ -52,14 +52,6 @@ RSX_HD bool ball_scored_now(const Ball& b) {
     return fabsf(b.xf.o.y * kBT2UU) > (kGoalThresholdY + kBallRadius);
 }
 
-// Vec::Normalized (MathTypes.h:88-95): divide by the length unless it is ~0.
-RSX_HD V3 rl_normalized(V3 v) {
-    float l2 = v.x * v.x + v.y * v.y + v.z * v.z;
-    float l = l2 > 0 ? sqrtf(l2) : 0.f;
-    if (l > 1.1920929e-07f * 1.1920929e-07f) return V3{v.x / l, v.y / l, v.z / l};
-    return V3{0, 0, 0};
-}
-
 RSX_HD float rl_dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 // One player's 19-float block (AddPlayerToObs), inverted for orange viewers.
@@ -152,12 +144,12 @@ RSX_HD float stub_reward(const Arena<MAXC>& A, int i, bool goal, bool touched, c
     if (E.touch_w != 0.f) r += (touched ? 1.f : 0.f) * E.touch_w;
     if (E.vbtg_w != 0.f) {
         V3 target = c.team == 0 ? V3{0, kBackNetY, kGoalHeightRL / 2} : V3{0, -kBackNetY, kGoalHeightRL / 2};
-        V3 dir = rl_normalized(target - bp);
+        V3 dir = rl_norm(target - bp);
         V3 nv = V3{bv.x / kRLBallMaxSpeed, bv.y / kRLBallMaxSpeed, bv.z / kRLBallMaxSpeed};
         r += rl_dot(dir, nv) * E.vbtg_w;
     }
     if (E.vptb_w != 0.f) {
-        V3 dir = rl_normalized(bp - cp);
+        V3 dir = rl_norm(bp - cp);
         V3 nv = V3{cv.x / kRLCarMaxSpeed, cv.y / kRLCarMaxSpeed, cv.z / kRLCarMaxSpeed};
         r += rl_dot(dir, nv) * E.vptb_w;
     }

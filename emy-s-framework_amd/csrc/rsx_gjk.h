@@ -1227,5 +1227,67 @@ RSX_HD bool closest_points(GjkEpa& G, const Shapes& s, const Xf& trA, const Xf& 
     return false;
 }
 
+// ---- ray vs convex (btSubsimplexConvexCast) ----------------------------------
+// btCollisionWorld::rayTestSingleInternal casts a zero-radius sphere from
+// `from` to `to` against a convex body with btSubsimplexConvexCast
+// (NarrowPhaseCollision/btSubSimplexConvexCast.cpp:38-140; single precision:
+// 32 iterations, epsilon 1e-4).  Shape: a box (kind 0, support of the box
+// grown by its margin, btBoxShape::localGetSupportingVertex) or a sphere
+// (kind 1, radius along the normalised direction).
+struct RayShape {
+    int kind;
+    V3 h;          // box: implicit half extents + margin; sphere: radius in h.x
+    Xf xf;         // world transform of the shape
+};
+
+RSX_HD V3 ray_shape_support(const RayShape& s, V3 d) {
+    if (s.kind == 0) return box_support(s.h, d);
+    V3 n = d;
+    if (len2(n) < kEps * kEps) n = V3{-1.f, -1.f, -1.f};
+    n = normalized(n);
+    return V3{0, 0, 0} + s.h.x * n;
+}
+
+// Returns true with the hit fraction and the (unnormalised) normal.
+RSX_HD bool ray_cast(const RayShape& s, V3 from, V3 to, float& fraction, V3& normal) {
+    Simplex sx;
+    sx_reset(sx);
+    const V3 r = to - from;  // linVelA - linVelB, the shape does not move
+    float lambda = 0.f;
+    V3 ia = from;
+    V3 supA = ia;
+    V3 supB = xform(s.xf, ray_shape_support(s, r * s.xf.b));
+    V3 v = supA - supB;
+    int max_iter = 32;
+    V3 n = V3{0, 0, 0};
+    float dist2 = len2(v);
+    while ((dist2 > 0.0001f) && max_iter--) {
+        supA = ia;
+        supB = xform(s.xf, ray_shape_support(s, v * s.xf.b));
+        V3 w = supA - supB;
+        float vdotw = dot(v, w);
+        if (lambda > 1.0f) return false;
+        if (vdotw > 0.f) {
+            float vdotr = dot(v, r);
+            if (vdotr >= -(kEps * kEps)) return false;
+            lambda = lambda - vdotw / vdotr;
+            float t = 1.0f - lambda;
+            ia = V3{t * from.x + lambda * to.x, t * from.y + lambda * to.y, t * from.z + lambda * to.z};
+            w = supA - supB;
+            n = v;
+        }
+        if (!sx_in(sx, w)) sx_add(sx, w, supA, supB);
+        if (sx_update(sx))
+            dist2 = len2(sx.cV), v = sx.cV;
+        else
+            dist2 = 0.f, v = sx.cV;
+    }
+    fraction = lambda;
+    V3 nn = (len2(n) >= kEps * kEps) ? normalized(n) : V3{0, 0, 0};
+    if (dot(nn, r) >= -0.f) return false;
+    normal = nn;
+    return true;
+}
+
 }  // namespace gjk
 }  // namespace rsx

@@ -73,6 +73,14 @@ RSX_HD V3 safe_normalized(V3 a) {
     if (l2 >= kEps * kEps) return a / sqrtf(l2);
     return V3{1.f, 0.f, 0.f};
 }
+// RocketSim's Vec::Normalized (Math/MathTypes/MathTypes.h:88-95): divide
+// each component by the length unless the length is ~0.
+RSX_HD V3 rl_norm(V3 v) {
+    float l2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    float l = l2 > 0 ? sqrtf(l2) : 0.f;
+    if (l > 1.1920929e-07f * 1.1920929e-07f) return V3{v.x / l, v.y / l, v.z / l};
+    return V3{0, 0, 0};
+}
 RSX_HD float dist2(V3 a, V3 b) { return len2(b - a); }
 
 RSX_HD V3 vmin(V3 a, V3 b) {

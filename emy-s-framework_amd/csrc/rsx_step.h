@@ -5,6 +5,7 @@
 // boost pads, ball limits and the goal test.
 #pragma once
 
+#include "rsx_boxbox.h"
 #include "rsx_gjk.h"
 #include "rsx_solver.h"
 
@@ -117,6 +118,53 @@ RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx
             V3 added = (hit_dir * rel_speed) * curve_eval(curve_ball_car_extra(), rel_speed) * 1.f;
             c.ball_hit_extra_vel = added;
             ball.vel_cache += added * kUU2BT;
+        }
+        return;
+    }
+    if (b0 >= 1 && b0 <= MAXC && b1 >= 1 && b1 <= MAXC) {
+        // car + car (Arena.cpp:336-418): manifold body A is "car1"
+        cp.friction = kCarCarFriction;
+        cp.restitution = kCarCarRestitution;
+        int i1 = b0 - 1, i2 = b1 - 1;
+        for (int pass = 0; pass < 2; pass++) {
+            const bool swapped = pass == 1;
+            if (swapped) {
+                int t = i1;
+                i1 = i2;
+                i2 = t;
+            }
+            Car& c1 = A.car[i1];
+            Car& c2 = A.car[i2];
+            if (c1.is_demoed || c2.is_demoed) return;
+            if (c1.car_contact_other_id == c2.id && c1.car_contact_cooldown > 0) continue;
+            const V3 pos1 = c1.xf.o * kBT2UU, pos2 = c2.xf.o * kBT2UU;
+            const V3 vel1 = c1.vel * kBT2UU, vel2 = c2.vel * kBT2UU;
+            const V3 delta = pos2 - pos1;
+            if (dot(vel1, delta) > 0) {
+                const V3 vel_dir = rl_norm(vel1);
+                const V3 dir_other = rl_norm(delta);
+                const float toward = dot(vel1, dir_other);
+                const float away = dot(vel2, vel_dir);
+                if (toward > away) {
+                    const V3 lp = swapped ? cp.localB : cp.localA;
+                    if ((lp.x * kBT2UU) > kBumpMinForwardDist) {
+                        bool demo = c1.is_supersonic;
+                        if (demo) demo = c1.team != c2.team;  // team demos off
+                        if (demo) {
+                            c2.is_demoed = true;
+                            c2.demo_respawn_timer = kDemoRespawnTime;
+                        } else {
+                            const bool ground = c2.on_ground;
+                            const float base = curve_eval(ground ? curve_bump_ground() : curve_bump_air(), toward);
+                            const V3 up = c2.on_ground ? col(c2.xf.b, 2) : V3{0, 0, 1};
+                            const V3 imp = vel_dir * base + up * curve_eval(curve_bump_up(), toward) * 1.f;
+                            c2.vel_cache += imp * kUU2BT;
+                        }
+                        c1.car_contact_other_id = c2.id;
+                        c1.car_contact_cooldown = kBumpCooldownTime;
+                    }
+                }
+            }
         }
         return;
     }
@@ -271,7 +319,10 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     for (int i = 0; i < A.ncars; i++) active[1 + i] = A.car[i].active;
 
     for (int d = 0; d < nb; d++) {
-        bool demoed = (d >= 1) && A.car[d - 1].is_demoed;
+        // a car demolished at the start of the tick has simulation and contact
+        // response disabled; one demolished during this tick's collision
+        // callbacks keeps both until the next tick (Car::_PreTickUpdate)
+        bool demoed = (d >= 1) && !A.car[d - 1].active;
         if (demoed) continue;  // no contact response: never solved
         const int cell = bp_cell_index(T.mv, W.bmin[d]);
         const uint16_t mask = T.mv.bp_meshes[cell];
@@ -372,7 +423,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         }
         // dynamic pairs with later bodies
         for (int e = d + 1; e < nb; e++) {
-            bool edemo = (e >= 1) && A.car[e - 1].is_demoed;
+            bool edemo = (e >= 1) && !A.car[e - 1].active;
             if (edemo) continue;
             if (!aabb_overlap(W.bmin[d], W.bmax[d], W.bmin[e], W.bmax[e])) continue;
             // islands: unite(d, e)
@@ -406,7 +457,36 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 }
                 mf_close(L, k);
             }
-            // car vs car contacts: not generated yet
+            if (d >= 1) {
+                // car d-1 vs car e-1: compound (car d) -> swapped compound
+                // (car e) -> box-box; the manifold's body A is the later car
+                const Car& cd = A.car[d - 1];
+                const Car& ce = A.car[e - 1];
+                Xf xd = car_child_xf(g, cd.xf), xe = car_child_xf(g, ce.xf);
+                V3 dmn, dmx, emn, emx;
+                box_child_aabb(g, xd, dmn, dmx);
+                car_shape_aabb(g, ce.xf, emn, emx);
+                if (!aabb_overlap(dmn, dmx, emn, emx)) continue;
+                box_child_aabb(g, xe, emn, emx);
+                if (!aabb_overlap(emn, emx, dmn, dmx)) continue;
+                int k = mf_open(L, e, d, g.breaking);
+                int s = k >= 0 ? L.slot[k] : -1;
+                if (s >= 0) {
+                    Manifold& m = L.live[s];
+                    const V3 side = (g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin}) * 2.f;
+                    boxbox::Out bo;
+                    bo.n = 0;
+                    boxbox::box_box(xe.o, xe.b, side, xd.o, xd.b, side, bo);
+                    for (int q = 0; q < bo.n; q++) {
+                        int idx = manifold_add(m, ce.xf, cd.xf, bo.normal_on_b[q], bo.point[q], bo.depth[q],
+                                               kCarCollisionFriction * kCarCollisionFriction,
+                                               kCarCollisionRestitution * kCarCollisionRestitution);
+                        if (idx >= 0) contact_added(A, T, m, idx);
+                    }
+                    manifold_refresh(m, ce.xf, cd.xf);
+                }
+                mf_close(L, k);
+            }
         }
     }
 
@@ -419,7 +499,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     for (int i = 0; i < nb; i++)
         if (active[i]) island_awake[tag[i]] = true;
     bool in_solver[MAXC + 1];
-    for (int i = 0; i < nb; i++) in_solver[i] = island_awake[tag[i]] && !((i >= 1) && A.car[i - 1].is_demoed);
+    for (int i = 0; i < nb; i++) in_solver[i] = island_awake[tag[i]] && !((i >= 1) && !A.car[i - 1].active);
     bool ball_woken = A.ball.sleeping && in_solver[0];
 
     int keys[kMaxManifolds], order[kMaxManifolds];
@@ -616,9 +696,15 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
             pad.cur_locked = 0;
         }
     }
+    M3 rot0[MAXC];
+    for (int i = 0; i < A.ncars; i++) rot0[i] = A.car[i].xf.b;
     arena_world_step(A, T, W);
     for (int i = 0; i < A.ncars; i++) {
         Car& c = A.car[i];
+        // A car demolished during this step skips Car::_PostTickUpdate, which
+        // is where its public rotation would have followed the body; the body
+        // is disabled from the next tick on, so only that rotation is kept.
+        if (c.is_demoed && c.active) c.xf.b = rot0[i];
         if (c.is_demoed) continue;
         car_post_tick(c, dt);
         car_finish_tick(c);

@@ -20,6 +20,8 @@
 // directory scan would leave to the filesystem.
 
 #include "RocketSim.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btTriangleInfoMap.h"
+#include "../libsrc/bullet3-3.24/BulletCollision/CollisionShapes/btStridingMeshInterface.h"
 #include "../include/rsx_types.h"
 
 #include <cstdio>
@@ -369,17 +371,80 @@ void ref_get_aabb(void* p, int i, float* out6) {
 }  // extern "C"
 
 // World ray test (btCollisionWorld::rayTest with a closest-hit callback) from
-// a to b in Bullet units; out: hit(1/0), fraction, point(3), normal(3), body code.
-extern "C" void ref_raytest(void* p, const float* a, const float* b, float* out) {
+// a to b in Bullet units, skipping car `ignore_car` (-1: none); out: hit(1/0),
+// fraction, point(3), normal(3), body code.
+extern "C" void ref_raytest(void* p, const float* a, const float* b, int ignore_car, float* out) {
     auto* h = (RefArena*)p;
     btVector3 from(a[0], a[1], a[2]), to(b[0], b[1], b[2]);
-    btCollisionWorld::ClosestRayResultCallback cb(from, to, nullptr);
+    const btCollisionObject* ign = ignore_car >= 0 ? &h->cars[ignore_car]->_rigidBody : nullptr;
+    btCollisionWorld::ClosestRayResultCallback cb(from, to, ign);
     h->arena->_bulletWorld.rayTest(from, to, cb);
     out[0] = cb.hasHit() ? 1.f : 0.f;
     out[1] = cb.m_closestHitFraction;
     out[2] = cb.m_hitPointWorld.x(); out[3] = cb.m_hitPointWorld.y(); out[4] = cb.m_hitPointWorld.z();
     out[5] = cb.m_hitNormalWorld.x(); out[6] = cb.m_hitNormalWorld.y(); out[7] = cb.m_hitNormalWorld.z();
     out[8] = cb.hasHit() ? (float)body_code(h, cb.m_collisionObject) : -1.f;
+}
+
+// Internal-edge table of soccar collision mesh `mesh` as Bullet built it
+// (btGenerateInternalEdgeInfo): per triangle 13 floats, the three vertices
+// then the V0V1 / V1V2 / V2V0 angles and the flags (bit pattern).  Returns
+// the triangle count; writes at most max_tris rows.
+extern "C" int ref_mesh_tri_info(int mesh, float* out, int max_tris) {
+    auto& shapes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (mesh < 0 || mesh >= (int)shapes.size()) return -1;
+    btBvhTriangleMeshShape* sh = shapes[mesh];
+    const btTriangleInfoMap* map = sh->getTriangleInfoMap();
+    const unsigned char* verts; int nverts; int stride;
+    const unsigned char* idx; int istride; int nfaces;
+    sh->getMeshInterface()->getLockedReadOnlyVertexIndexBase(&verts, nverts, stride, &idx, istride, nfaces, 0);
+    for (int t = 0; t < nfaces && t < max_tris; t++) {
+        float* o = out + (size_t)t * 13;
+        const int* tri = (const int*)(idx + (size_t)t * istride);
+        for (int k = 0; k < 3; k++) {
+            const float* v = (const float*)(verts + (size_t)tri[k] * stride);
+            o[k * 3 + 0] = v[0]; o[k * 3 + 1] = v[1]; o[k * 3 + 2] = v[2];
+        }
+        btTriangleInfo info;
+        const btTriangleInfo* f = map ? map->find(btHashInt(t)) : nullptr;
+        if (f) info = *f;
+        o[9] = info.m_edgeV0V1Angle; o[10] = info.m_edgeV1V2Angle; o[11] = info.m_edgeV2V0Angle;
+        memcpy(&o[12], &info.m_flags, 4);
+    }
+    sh->getMeshInterface()->unLockReadOnlyVertexBase(0);
+    return nfaces;
+}
+
+// Ray test of one convex body (btCollisionWorld::rayTestSingle with a
+// closest-hit callback): kind 0 a box with implicit half extents dims[0..2]
+// and margin dims[3], kind 1 a sphere of radius dims[0].  xf12: row-major
+// basis (9) then origin (3).  out: hit(1/0), fraction, normal(3).
+extern "C" void ref_ray_convex(int kind, const float* dims, const float* xf12, const float* a, const float* b,
+                               float* out) {
+    btBoxShape box(btVector3(dims[0] + 0.04f, dims[1] + 0.04f, dims[2] + 0.04f));
+    btSphereShape sphere(dims[0]);
+    btCollisionShape* shape = &sphere;
+    if (kind == 0) {
+        box.btConvexInternalShape::setMargin(dims[3]);
+        shape = &box;
+    }
+    btTransform t;
+    t.setBasis(btMatrix3x3(xf12[0], xf12[1], xf12[2], xf12[3], xf12[4], xf12[5], xf12[6], xf12[7], xf12[8]));
+    t.setOrigin(btVector3(xf12[9], xf12[10], xf12[11]));
+    btCollisionObject obj;
+    obj.setCollisionShape(shape);
+    obj.setWorldTransform(t);
+    btVector3 from(a[0], a[1], a[2]), to(b[0], b[1], b[2]);
+    btTransform tf, tt;
+    tf.setIdentity();
+    tf.setOrigin(from);
+    tt.setIdentity();
+    tt.setOrigin(to);
+    btCollisionWorld::ClosestRayResultCallback cb(from, to, nullptr);
+    btCollisionWorld::rayTestSingle(tf, tt, &obj, shape, t, cb);
+    out[0] = cb.hasHit() ? 1.f : 0.f;
+    out[1] = cb.m_closestHitFraction;
+    out[2] = cb.m_hitNormalWorld.x(); out[3] = cb.m_hitNormalWorld.y(); out[4] = cb.m_hitNormalWorld.z();
 }
 
 // CPU baseline loop for the benchmark: `steps` env steps over the arenas

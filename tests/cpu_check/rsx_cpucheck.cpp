@@ -210,6 +210,43 @@ extern "C" void cc_raytest_static(void* w, const float* a, const float* b, float
     out[5] = hit.normal.x; out[6] = hit.normal.y; out[7] = hit.normal.z;
 }
 
+// The engine's triangle table: per triangle 14 floats, the three vertices,
+// the internal-edge angles and flags (bit pattern), and the mesh index.
+extern "C" int cc_tri_info(void* w, float* out, int max_tris) {
+    auto* W = (CcWorld*)w;
+    const HostWorld& h = W->w;
+    int n = (int)h.tri_mesh.size();
+    for (int t = 0; t < n && t < max_tris; t++) {
+        float* o = out + (size_t)t * 14;
+        memcpy(o, &h.tri[(size_t)t * 9], 9 * sizeof(float));
+        memcpy(o + 9, &h.tri_info[(size_t)t * 4], 4 * sizeof(float));
+        o[13] = (float)h.tri_mesh[t];
+    }
+    return n;
+}
+
+// Ray test of one convex body, as the suspension ray test does it for the
+// ball and the car boxes (same arguments and outputs as the oracle's
+// ref_ray_convex).
+extern "C" void cc_ray_convex(int kind, const float* dims, const float* xf12, const float* a, const float* b,
+                              float* out) {
+    gjk::RayShape s;
+    s.kind = kind;
+    s.h = kind == 0 ? V3{dims[0], dims[1], dims[2]} + V3{dims[3], dims[3], dims[3]} : V3{dims[0], 0, 0};
+    s.xf.b = m3(xf12[0], xf12[1], xf12[2], xf12[3], xf12[4], xf12[5], xf12[6], xf12[7], xf12[8]);
+    s.xf.o = V3{xf12[9], xf12[10], xf12[11]};
+    float f;
+    V3 n;
+    out[0] = 0.f;
+    out[1] = 1.f;
+    if (gjk::ray_cast(s, V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}, f, n) && len2(n) > 0.0001f && f < 1.0f) {
+        n = normalized(n);
+        out[0] = 1.f;
+        out[1] = f;
+        out[2] = n.x; out[3] = n.y; out[4] = n.z;
+    }
+}
+
 // Env outputs of the current state after a step of tick_skip ticks, with the
 // default stub-reward weights (1, concede -1, 1, 1, 1): each car's DefaultObs
 // row into obs (rows back to back), rewards, and the terminal type (return

@@ -94,6 +94,8 @@ def load_oracle():
         ("ref_pad_config", None, [VP, C.POINTER(C.c_float)]),
         ("ref_sizeof_car_state", C.c_uint32, []),
         ("ref_num_cars", C.c_int, [VP]),
+        ("ref_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
+        ("ref_mesh_tri_info", C.c_int, [C.c_int, C.POINTER(C.c_float), C.c_int]),
     ])
     if lib is not None:
         assert lib.ref_init(MESH_DIR.encode()) == 0
@@ -120,6 +122,8 @@ def load_cpucheck():
         ("cc_get_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
         ("cc_step", C.c_int, [VP, C.c_int]),
         ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+        ("cc_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
+        ("cc_tri_info", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
         ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ])

@@ -76,4 +76,29 @@ def cases():
             P.default_car(pos=(0, 4240 - 400, 17), yaw=np.pi / 2, vel=(0, 1000, 0), team=1, boost=10)]
     c["boost pads"] = dict(blue=1, orange=1, cars=pads, ball=P.ball(),
                            controls=[P.controls(throttle=1), P.controls(throttle=1)], ticks=60)
+    # car box against the arena mesh (GJK / EPA per triangle, internal edges)
+    other = P.default_car(pos=(3000, 4000, 17), yaw=np.pi, team=1)
+    drive = [P.controls(throttle=1), P.controls()]
+    c["car into back wall"] = dict(blue=1, orange=1, ball=P.ball(), controls=drive, ticks=90, cars=[
+        P.default_car(pos=(1500, -4500, 17), yaw=-np.pi / 2 + 0.2, vel=(300, -1400, 0)), other])
+    c["car up side ramp"] = dict(blue=1, orange=1, ball=P.ball(), controls=drive, ticks=90, cars=[
+        P.default_car(pos=(3300, 0, 17), yaw=0.1, vel=(1500, 100, 0)), other])
+    c["car falls on corner"] = dict(blue=1, orange=1, ball=P.ball(), controls=_idle(), ticks=90, cars=[
+        P.car_from_rot((3700, -4700, 300), np.eye(3), vel=(200, -200, -300), ang=(1, 2, 0.5)), other])
+    c["car roof on ramp"] = dict(blue=1, orange=1, ball=P.ball(), controls=_idle(), ticks=90, cars=[
+        P.car_from_rot((3500, 1000, 200), np.diag([1, -1, -1]).astype(float), vel=(600, 0, -200)), other])
+    # car against car (box-box manifold, bump and demolition callback)
+    far_ball = P.ball((2000, 0, 93))
+    c["head-on bump"] = dict(blue=1, orange=1, ball=far_ball, ticks=60,
+                             controls=[P.controls(throttle=1), P.controls(throttle=1)], cars=[
+        P.default_car(pos=(0, -300, 17), yaw=np.pi / 2, vel=(0, 1000, 0)),
+        P.default_car(pos=(0, 300, 17), yaw=-np.pi / 2 + 0.1, vel=(0, -600, 0), team=1)])
+    c["side bump"] = dict(blue=1, orange=1, ball=P.ball((2000, 2000, 93)), ticks=60,
+                          controls=[P.controls(throttle=1), P.controls(throttle=1)], cars=[
+        P.default_car(pos=(-400, 0, 17), yaw=0.0, vel=(1500, 0, 0)),
+        P.default_car(pos=(100, 30, 17), yaw=np.pi / 2, vel=(0, 300, 0), team=1)])
+    c["demolition"] = dict(blue=1, orange=1, ball=far_ball, ticks=60,
+                           controls=[P.controls(throttle=1, boost=1), P.controls()], cars=[
+        P.default_car(pos=(0, -600, 17), yaw=np.pi / 2, vel=(0, 2250, 0), boost=100),
+        P.default_car(pos=(30, 300, 17), yaw=0.3, team=1)])
     return c
