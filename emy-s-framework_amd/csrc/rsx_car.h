@@ -102,14 +102,27 @@ RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
 // covers every body a suspension-length ray from the origin's cell can reach.
 template <int MAXC>
 RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int self,
-                      V3 from, V3 to, RayHit& hit) {
+                      V3 from, V3 to, RayHit& hit, gjk::Simplex& sx) {
     hit.object = -1;
     hit.fraction = 1.0f;
     ray_static_world(mv, from, to, hit);
+    const V3 seg = to - from;
+    const float seg2 = dot(seg, seg);
+    // A cast only reports a hit where the ray comes within its stopping
+    // distance (1e-2) of the body, so a body whose bounding sphere stays
+    // farther than that from the whole segment is skipped; 0.25 keeps a wide
+    // safety band around the stopping distance.
+    auto far_from = [&](V3 centre, float radius) {
+        float t = seg2 > 0.f ? clampf(dot(centre - from, seg) / seg2, 0.f, 1.f) : 0.f;
+        V3 p = from + seg * t;
+        float r = radius + 0.25f;
+        return len2(centre - p) > r * r;
+    };
     auto try_body = [&](const gjk::RayShape& s, int object) {
         float f;
         V3 n;
-        if (!gjk::ray_cast(s, from, to, f, n)) return;
+        if (far_from(s.xf.o, s.kind == 0 ? len(s.h) : s.h.x)) return;
+        if (!gjk::ray_cast(s, from, to, f, n, sx)) return;
         if (!(len2(n) > 0.0001f)) return;
         float cur = hit.object >= 0 ? hit.fraction : 1.0f;
         if (!(f < cur)) return;
@@ -138,7 +151,7 @@ RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g
 // btVehicleRL::rayCast (btVehicleRL.cpp:118-212)
 template <int MAXC>
 RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, int i,
-                          float solver_dt) {
+                          float solver_dt, gjk::Simplex& sx) {
     Car& c = A.car[ci];
     wheel_update_ws(c, g, i);
     Wheel& w = c.wheel[i];
@@ -151,7 +164,7 @@ RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGe
     w.contact_point = target;
     w.ground = -1;
     RayHit hit;
-    ray_world(mv, A, g, bg, ci, source, target, hit);
+    ray_world(mv, A, g, bg, ci, source, target, hit, sx);
     if (hit.object >= 0) {
         float s = 1.0f - hit.fraction;
         w.contact_point = V3{s * source.x + hit.fraction * target.x, s * source.y + hit.fraction * target.y,
@@ -640,11 +653,11 @@ RSX_HD void car_update_auto_roll(Car& c, const CarGeom& g, float dt, int nwc) {
 // Car::_PreTickUpdate for a car that is not demoed (demo handling is done by the caller).
 template <int MAXC>
 RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, float dt,
-                         float solver_dt) {
+                         float solver_dt, gjk::Simplex& sx) {
     Car& c = A.car[ci];
     // updateVehicleFirst
     for (int i = 0; i < 4; i++) wheel_update_transform(c, g, i);
-    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt);
+    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx);
     vehicle_friction_impulses(A, c, g, dt, bg);
 
     bool jump_pressed = c.controls.jump && !c.last_controls.jump;

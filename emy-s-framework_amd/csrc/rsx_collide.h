@@ -41,6 +41,7 @@ struct TickCtx {
     MeshView mv;
     const PadGeom* pads;
     float dt;
+    gjk::Simplex* sx;  // this lane's Voronoi simplex working set (LDS in the kernels)
 };
 
 // --- manifold bookkeeping ----------------------------------------------------

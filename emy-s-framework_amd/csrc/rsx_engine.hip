@@ -107,8 +107,12 @@ __host__ __device__ void reset_column(const Col& col, const CarGeom& cg) {
     col.set(L::kArenaBase + kArEpisode, col.get(L::kArenaBase + kArEpisode) + 1u);
 }
 
+// Lanes per workgroup of the arena kernels (one wavefront; one lane per
+// arena).  Their LDS arrays are sized by it.
+constexpr int kArenaBlock = 64;
+
 template <int MAXC>
-__global__ void __launch_bounds__(64) k_step(uint32_t* words, int n, const rsx_controls* ctrl, int ticks,
+__global__ void __launch_bounds__(kArenaBlock) k_step(uint32_t* words, int n, const rsx_controls* ctrl, int ticks,
                                               CarGeom cg, BallGeom bg, MeshView mv, const PadGeom* pads) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
@@ -126,6 +130,10 @@ __global__ void __launch_bounds__(64) k_step(uint32_t* words, int n, const rsx_c
     T.mv = mv;
     T.pads = pads;
     T.dt = 1.f / kTickRate;
+    // the GJK / convex-cast simplex is read and written many times per query:
+    // keep it in LDS rather than in the lane's scratch
+    __shared__ gjk::Simplex sx_lds[kArenaBlock];
+    T.sx = &sx_lds[threadIdx.x];
     TickScratch<MAXC> W;
     uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
@@ -150,7 +158,7 @@ __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint
 // stub rewards, the terminal conditions, the kickoff auto-reset of terminal
 // arenas and the DefaultObs rows, all in the lane that owns the arena.
 template <int MAXC>
-__global__ void __launch_bounds__(64) k_env_step(uint32_t* words, int n, const int32_t* act, const int* car_off,
+__global__ void __launch_bounds__(kArenaBlock) k_env_step(uint32_t* words, int n, const int32_t* act, const int* car_off,
                                                  const rsx_controls* lut, int lut_n, int ticks, CarGeom cg,
                                                  BallGeom bg, MeshView mv, const PadGeom* pads, EnvConfig E,
                                                  float* obs, const int64_t* obs_off, float* rew, uint8_t* term,
@@ -184,6 +192,10 @@ __global__ void __launch_bounds__(64) k_env_step(uint32_t* words, int n, const i
     T.mv = mv;
     T.pads = pads;
     T.dt = 1.f / kTickRate;
+    // the GJK / convex-cast simplex is read and written many times per query:
+    // keep it in LDS rather than in the lane's scratch
+    __shared__ gjk::Simplex sx_lds[kArenaBlock];
+    T.sx = &sx_lds[threadIdx.x];
     TickScratch<MAXC> W;
     uint32_t flags = col.get(L::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
@@ -900,7 +912,7 @@ rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stre
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
         if (!b.n) continue;
-        dim3 grid((b.n + 63) / 64), block(64);
+        dim3 grid((b.n + kArenaBlock - 1) / kArenaBlock), block(kArenaBlock);
 #define RSX_LAUNCH_ENV(M)                                                                                        \
     hipLaunchKernelGGL(k_env_step<M>, grid, block, 0, st, b.d_words, b.n, device_actions, b.d_car_off, e->d_lut, \
                        e->lut_n, e->tick_skip, e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->env, e->d_obs,       \
@@ -984,7 +996,7 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
     if (ticks < 0) return fail(RSX_INVALID_ARG, "ticks must be >= 0");
     HIP_TRY(hipSetDevice(e->device));
     hipStream_t st = (hipStream_t)stream;
-    const int bs = 64;
+    const int bs = kArenaBlock;
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
         if (!b.n) continue;

@@ -1066,7 +1066,9 @@ struct Contact1 {
 
 // btGjkPairDetector::getClosestPointsNonVirtual for box (A) vs triangle (B).
 // Returns true with one contact when the reference would report one.
-RSX_HD bool closest_points(GjkEpa& G, const Shapes& s, const Xf& trA, const Xf& trB, float max_dist2, Contact1& out) {
+// sx is the caller's Voronoi simplex working set (reset here).
+RSX_HD bool closest_points(GjkEpa& G, Simplex& sx, const Shapes& s, const Xf& trA, const Xf& trB, float max_dist2,
+                           Contact1& out) {
     float dist = 0.f;
     V3 normalInB = V3{0, 0, 0};
     V3 pointOnA = V3{0, 0, 0}, pointOnB = V3{0, 0, 0};
@@ -1083,7 +1085,6 @@ RSX_HD bool closest_points(GjkEpa& G, const Shapes& s, const Xf& trA, const Xf& 
     float squaredDistance = kLargeFloat;
     float delta = 0.f;
     const float margin = marginA + marginB;
-    Simplex sx;
     sx_reset(sx);
     while (true) {
         V3 sepA = (-sep) * trA.b;  // (-v) * basisA
@@ -1248,9 +1249,9 @@ RSX_HD V3 ray_shape_support(const RayShape& s, V3 d) {
     return V3{0, 0, 0} + s.h.x * n;
 }
 
-// Returns true with the hit fraction and the (unnormalised) normal.
-RSX_HD bool ray_cast(const RayShape& s, V3 from, V3 to, float& fraction, V3& normal) {
-    Simplex sx;
+// Returns true with the hit fraction and the (unnormalised) normal; sx is
+// the caller's Voronoi simplex working set.
+RSX_HD bool ray_cast(const RayShape& s, V3 from, V3 to, float& fraction, V3& normal, Simplex& sx) {
     sx_reset(sx);
     const V3 r = to - from;  // linVelA - linVelB, the shape does not move
     float lambda = 0.f;

@@ -344,6 +344,11 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
         a[3] = amax.x; a[4] = amax.y; a[5] = amax.z;
     }
     const int ntri = (int)w.tri_mesh.size();
+    if (ntri > kMaxTris) {
+        err = "collision meshes hold " + std::to_string(ntri) + " triangles; the candidate packing allows " +
+              std::to_string(kMaxTris);
+        return false;
+    }
 
     // Uniform grid over all triangles (4 BT cells).
     V3 gmin = V3{kLargeFloat, kLargeFloat, kLargeFloat}, gmax = -gmin;
@@ -533,7 +538,7 @@ static void internal_edge_info(HostWorld& w) {
         float info[4] = {two_pi, two_pi, two_pi, 0.f};
         bool any = false;
         for (int k = 0; k < n; k++) {
-            int u = cand[k];
+            int u = cand_tri(cand[k]);
             if (u == t || w.tri_mesh[u] != w.tri_mesh[t]) continue;
             V3 B[3];
             load_tri(mv, u, B[0], B[1], B[2]);

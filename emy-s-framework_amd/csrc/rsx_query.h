@@ -53,9 +53,20 @@ RSX_HD void load_tri(const MeshView& w, int t, V3& a, V3& b, V3& c) {
 }
 
 RSX_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+RSX_HD int maxi(int a, int b) { return a > b ? a : b; }
 
-// Candidate triangles overlapping [mn, mx], de-duplicated, sorted by
-// (mesh index, tree traversal rank).  Returns the count (capped at cap).
+// A gathered candidate packs the triangle's mesh (bits 26-30), its rank in
+// that mesh's tree traversal (bits 13-25) and its index (bits 0-12), so that
+// sorting the packed values orders candidates by (mesh, rank).
+constexpr int kCandTriBits = 13, kCandRankShift = 13, kCandMeshShift = 26;
+constexpr int kMaxTris = 1 << kCandTriBits;
+RSX_HD int cand_tri(int p) { return p & (kMaxTris - 1); }
+RSX_HD int cand_mesh(int p) { return p >> kCandMeshShift; }
+
+// Candidate triangles overlapping [mn, mx], each once, sorted by (mesh index,
+// tree traversal rank); see cand_tri / cand_mesh.  Returns the count (capped
+// at cap).  A triangle is listed in every grid cell its AABB touches, so it is
+// taken only from the first of those cells that the query visits.
 RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
     const float inv = 1.0f / w.grid_cell;
     int x0 = clampi((int)floorf((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
@@ -72,27 +83,27 @@ RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
                 int s = w.cell_start[cell], e = w.cell_start[cell + 1];
                 for (int k = s; k < e; k++) {
                     int t = w.cell_tris[k];
-                    // de-duplicate: keep only if not already present
-                    bool dup = false;
-                    for (int j = 0; j < n; j++)
-                        if (out[j] == t) { dup = true; break; }
-                    if (dup) continue;
                     V3 a, b, c;
                     load_tri(w, t, a, b, c);
+                    V3 tmn = vmin(vmin(a, b), c);
+                    // first cell of the triangle's own range inside the query range
+                    int fx = maxi(x0, clampi((int)floorf((tmn.x - w.grid_min.x) * inv), 0, w.gx - 1));
+                    int fy = maxi(y0, clampi((int)floorf((tmn.y - w.grid_min.y) * inv), 0, w.gy - 1));
+                    int fz = maxi(z0, clampi((int)floorf((tmn.z - w.grid_min.z) * inv), 0, w.gz - 1));
+                    if (fx != ix || fy != iy || fz != iz) continue;
                     if (!tri_aabb_overlap(a, b, c, mn, mx)) continue;
-                    if (n < cap) out[n++] = t;
+                    if (n < cap) out[n++] = (w.tri_mesh[t] << kCandMeshShift) | (w.tri_rank[t] << kCandRankShift) | t;
                 }
             }
-    // insertion sort by (mesh, rank)
+    // insertion sort of the packed keys
     for (int i = 1; i < n; i++) {
-        int t = out[i];
-        int key = w.tri_mesh[t] * 65536 + w.tri_rank[t];
+        int key = out[i];
         int j = i - 1;
-        while (j >= 0 && (w.tri_mesh[out[j]] * 65536 + w.tri_rank[out[j]]) > key) {
+        while (j >= 0 && out[j] > key) {
             out[j + 1] = out[j];
             j--;
         }
-        out[j + 1] = t;
+        out[j + 1] = key;
     }
     return n;
 }
@@ -145,7 +156,7 @@ RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit) {
     float frac = hit.object >= 0 ? hit.fraction : 1.0f;
     for (int i = 0; i < n; i++) {
         V3 a, b, c;
-        load_tri(w, cand[i], a, b, c);
+        load_tri(w, cand_tri(cand[i]), a, b, c);
         V3 nrm;
         if (ray_triangle(from, to, a, b, c, frac, nrm)) {
             hit.fraction = frac;

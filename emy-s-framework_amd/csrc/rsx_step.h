@@ -177,8 +177,10 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
     const Ball& b = A.ball;
     Xf trB = xf_identity();
     for (int i = 0; i < ncand; i++) {
-        int t = cand[i];
-        if (T.mv.tri_mesh[t] != mesh) continue;
+        const int pk = cand[i];
+        if (cand_mesh(pk) < mesh) continue;
+        if (cand_mesh(pk) > mesh) break;
+        const int t = cand_tri(pk);
         V3 v0, v1, v2;
         load_tri(T.mv, t, v0, v1, v2);
         if (sphere_tri_early_out(b.xf.o, T.bg.radius, v0, v1, v2, m.breaking)) continue;
@@ -207,8 +209,10 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifo
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
     const float max_d = g.box_margin + 0.f + m.breaking;
     for (int i = 0; i < ncand; i++) {
-        int t = cand[i];
-        if (T.mv.tri_mesh[t] != mesh) continue;
+        const int pk = cand[i];
+        if (cand_mesh(pk) < mesh) continue;
+        if (cand_mesh(pk) > mesh) break;
+        const int t = cand_tri(pk);
         gjk::Shapes sh;
         load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
         // early out: the box is wholly on one side of the triangle's plane
@@ -227,7 +231,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifo
         sh.h = g.box_implicit;
         sh.margin = g.box_margin;
         gjk::Contact1 ct;
-        if (gjk::closest_points(G, sh, child, trB, max_d * max_d, ct)) {
+        if (gjk::closest_points(G, *T.sx, sh, child, trB, max_d * max_d, ct)) {
             int idx = manifold_add(m, c.xf, trB, ct.normal_on_b, ct.point_on_b, ct.depth, 0.6f, 0.3f);
             if (idx >= 0) contact_added(A, T, m, idx, t);
         }
@@ -686,7 +690,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
             c.active = false;  // body disabled for this tick
         }
         if (c.is_demoed) continue;
-        car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt);
+        car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx);
     }
     if (A.ncars > 0) {
         for (int p = 0; p < kNumPads; p++) {

@@ -134,6 +134,8 @@ int cc_step(void* p, int ticks) {
     T.mv = a->world->mv;
     T.pads = a->world->w.pads;
     T.dt = 1.f / kTickRate;
+    gjk::Simplex sx;
+    T.sx = &sx;
     int scored = 0;
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;
@@ -237,9 +239,10 @@ extern "C" void cc_ray_convex(int kind, const float* dims, const float* xf12, co
     s.xf.o = V3{xf12[9], xf12[10], xf12[11]};
     float f;
     V3 n;
+    gjk::Simplex sx;
     out[0] = 0.f;
     out[1] = 1.f;
-    if (gjk::ray_cast(s, V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}, f, n) && len2(n) > 0.0001f && f < 1.0f) {
+    if (gjk::ray_cast(s, V3{a[0], a[1], a[2]}, V3{b[0], b[1], b[2]}, f, n, sx) && len2(n) > 0.0001f && f < 1.0f) {
         n = normalized(n);
         out[0] = 1.f;
         out[1] = f;
