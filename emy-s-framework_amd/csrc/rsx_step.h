@@ -329,9 +329,27 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         bool demoed = (d >= 1) && !A.car[d - 1].active;
         if (demoed) continue;  // no contact response: never solved
         const int cell = bp_cell_index(T.mv, W.bmin[d]);
-        const uint16_t mask = T.mv.bp_meshes[cell];
-        // statics: meshes then planes
-        int ncand = -1;
+        uint32_t mask = T.mv.bp_meshes[cell];
+        // statics: meshes then planes.  The body's candidate triangles are
+        // gathered once; a mesh with none of them cannot add a contact, and its
+        // pair would only carry an empty manifold, so it is skipped.
+        int ncand = 0;
+        if (mask && active[d]) {
+            if (d == 0) {
+                V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
+                ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
+            } else {
+                Xf child = car_child_xf(g, A.car[d - 1].xf);
+                V3 cmn, cmx;
+                box_child_aabb(g, child, cmn, cmx);
+                ncand = gather_tris(T.mv, cmn, cmx, W.cand, 48);
+            }
+            uint32_t cmask = 0;
+            for (int i = 0; i < ncand; i++) cmask |= 1u << cand_mesh(W.cand[i]);
+            mask &= cmask;
+        } else {
+            mask = 0;
+        }
         for (int ms = 0; ms < kNumMeshes; ms++) {
             if (!((mask >> ms) & 1)) continue;
             const float* ma = T.mv.mesh_aabb + ms * 6;
@@ -352,21 +370,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
-                if (d == 0) {
-                    if (ncand < 0) {
-                        V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
-                        ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
-                    }
-                    ball_vs_mesh(A, T, m, ms, W.cand, ncand);
-                } else {
-                    if (ncand < 0) {
-                        Xf child = car_child_xf(g, A.car[d - 1].xf);
-                        V3 cmn, cmx;
-                        box_child_aabb(g, child, cmn, cmx);
-                        ncand = gather_tris(T.mv, cmn, cmx, W.cand, 48);
-                    }
-                    car_vs_mesh(A, T, W.gj, m, d - 1, ms, W.cand, ncand);
-                }
+                if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand);
+                else car_vs_mesh(A, T, W.gj, m, d - 1, ms, W.cand, ncand);
             }
             mf_close(L, k);
         }
