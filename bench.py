@@ -108,10 +108,14 @@ def cpu_baseline(seconds, threads, tick_skip=8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--arenas", type=int, default=65536)
     ap.add_argument("--tick-skip", type=int, default=8)
+    ap.add_argument("--workload", choices=("stress", "random"), default="stress")
+    ap.add_argument("--cull", choices=("auto", "on", "off"), default="auto")
+    ap.add_argument("--audit-steps", type=int, default=8)
+    ap.add_argument("--cull-every", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -122,12 +126,13 @@ def main():
 
     import torch
     eng = _import_package()
-    from emy_s_framework_amd.workloads import random_field_states
+    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf
+    from emy_s_framework_amd.workloads import random_field_states, stress_states
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
     batch = eng.RocketSimBatch(args.arenas, 1, tick_skip=args.tick_skip, seed=4, device=0)
-    random_field_states(batch, seed=4)
+    apply_states = stress_states if args.workload == "stress" else random_field_states
     batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
     n_cars = batch.num_cars
     stream = torch.cuda.current_stream(dev)
@@ -136,58 +141,85 @@ def main():
     gen.manual_seed(4)
     actions = [torch.randint(0, 90, (n_cars,), dtype=torch.int32, device=dev, generator=gen)
                for _ in range(8)]
-    torch.cuda.synchronize()
+    use_cull = args.cull == "on" or (args.cull == "auto" and os.path.exists(WEIGHTS_PATH))
+    net = None
+    margin = None
+    if use_cull:
+        net = NeuralSdf(batch, seed=0)
+        margin = float(net.load(WEIGHTS_PATH)["margin_uu"])
 
-    def step(i, ev=None):
-        a = actions[i % len(actions)]
-        if ev is not None:
-            ev[0].record(stream)
-        batch.StepEnv(a.data_ptr(), sp)
-        if ev is not None:
-            ev[1].record(stream)
+    def run(tag, steps, warmup):
+        """Fresh workload states, warmup, then `steps` timed fused env steps."""
+        apply_states(batch, seed=4)
+        torch.cuda.synchronize()
+        for i in range(warmup):
+            batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            evs[i][0].record(stream)
+            batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+            evs[i][1].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kern = sum(a.elapsed_time(b) for a, b in evs) / steps
+        print(f"# {tag}: {steps} steps, {wall * 1e3 / steps:.2f} ms/step (kernel {kern:.2f} ms)", file=sys.stderr,
+              flush=True)
+        return {"value": args.arenas * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern}
 
-    t0 = time.perf_counter()
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    print(f"# warmup: {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i, evs[i])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = sum(a.elapsed_time(b) for a, b in evs) / args.steps
-
-    value = args.arenas * args.steps / wall
-    achieved = ALGO_BYTES_1V1 * args.arenas / (kern_ms * 1e-3) / 1e9
+    exact = run("exact collision", args.steps, args.warmup)
+    cull = audit = None
+    if use_cull:
+        net.attach(margin, every=args.cull_every, audit=False)
+        cull = run("learned cull", args.steps, args.warmup)
+        net.attach(margin, every=args.cull_every, audit=True)
+        net.reset_stats()
+        run("learned cull, audited", args.audit_steps, 0)
+        audit = net.stats()
+        body_ticks = args.audit_steps * args.tick_skip * args.arenas * 3
+        audit.update({"steps": args.audit_steps, "margin_uu": margin, "every_ticks": args.cull_every,
+                      "culled_fraction": audit["culled"] / body_ticks})
+        net.detach()
+    head = cull if cull is not None else exact
+    achieved = ALGO_BYTES_1V1 * args.arenas / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds, max(1, min(16, os.cpu_count() or 1)))
+    wl = ("contact-heavy states (seed 4: 30% ball near corners/ramps/goal mouths, 20% cars on walls or "
+          "ceiling, 10% overlapping car spawns)" if args.workload == "stress" else "random field states (seed 4)")
     out = {
         "metric": "env-steps/sec @65536 1v1 arenas (neural-SDF cull vs exact), % HBM roofline",
-        "value": value,
+        "value": head["value"],
         "unit": "env-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.arenas} soccar 1v1 arenas, random field states (seed 4), uniform random "
-                               f"discrete actions, tick-skip {args.tick_skip}, exact collision, fused env step "
-                               f"(physics + DefaultObs + 4 stub rewards + terminals + auto-reset)",
-                   "arenas": args.arenas, "tick_skip": args.tick_skip, "cull": "none (exact collision)"},
+        "config": {"workload": f"{args.arenas} soccar 1v1 arenas, {wl}, uniform random discrete actions, "
+                               f"tick-skip {args.tick_skip}, fused env step (physics + DefaultObs + 4 stub rewards "
+                               f"+ terminals + auto-reset)",
+                   "arenas": args.arenas, "tick_skip": args.tick_skip,
+                   "cull": (f"learned SDF cull every {args.cull_every} ticks, margin {margin:.1f} uu, exact "
+                            f"fallback" if cull is not None else "none (exact collision)")},
+        "exact": exact,
+        "cull": cull,
+        "audit": audit,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_env_step<2>", "kernel_ms": kern_ms, "algo_bytes_per_env_step": ALGO_BYTES_1V1},
+                     "kernel": "k_env_step<2>", "kernel_ms": head["kernel_ms"],
+                     "algo_bytes_per_env_step": ALGO_BYTES_1V1},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if net is not None:
+        net.destroy()
     batch.Destroy()
 
 

@@ -56,6 +56,21 @@ _SIGS = [
     ("rsx_copy_terminals", _I, [_VP, C.POINTER(C.c_uint8), _I]),
     ("rsx_step_ticks", _I, [_VP, _I, _VP]),
     ("rsx_synchronize", _I, [_VP]),
+    # learned broadphase cull (include/rsx_sdf.h)
+    ("rsx_sdf_num_params", _I, []),
+    ("rsx_sdf_create", _I, [_VP, C.c_uint64, C.POINTER(_VP)]),
+    ("rsx_sdf_destroy", _I, [_VP]),
+    ("rsx_sdf_mesh_triangles", _I, [_VP, C.POINTER(C.c_float), C.c_int64, C.POINTER(_I)]),
+    ("rsx_sdf_exact_distance", _I, [_VP, _VP, _I, _VP, _VP]),
+    ("rsx_sdf_eval", _I, [_VP, _VP, _I, _VP, _VP]),
+    ("rsx_sdf_train_step", _I, [_VP, _VP, _VP, _I, C.c_float, C.c_float, C.c_float, _VP, _VP]),
+    ("rsx_sdf_gradients", _I, [_VP, _VP, _VP, _I, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    ("rsx_sdf_get_params", _I, [_VP, C.POINTER(C.c_float)]),
+    ("rsx_sdf_set_params", _I, [_VP, C.POINTER(C.c_float)]),
+    ("rsx_attach_cull", _I, [_VP, _VP, C.c_float, _I, _I]),
+    ("rsx_get_cull_stats", _I, [_VP, C.POINTER(C.c_uint64 * 3)]),
+    ("rsx_reset_cull_stats", _I, [_VP]),
+    ("rsx_cull_reach", _I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
 ]
 EXPORTED = [name for name, _, _ in _SIGS]
 

@@ -102,10 +102,10 @@ RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
 // covers every body a suspension-length ray from the origin's cell can reach.
 template <int MAXC>
 RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int self,
-                      V3 from, V3 to, RayHit& hit, gjk::Simplex& sx) {
+                      V3 from, V3 to, RayHit& hit, gjk::Simplex& sx, bool with_mesh) {
     hit.object = -1;
     hit.fraction = 1.0f;
-    ray_static_world(mv, from, to, hit);
+    ray_static_world(mv, from, to, hit, with_mesh);
     const V3 seg = to - from;
     const float seg2 = dot(seg, seg);
     // A cast only reports a hit where the ray comes within its stopping
@@ -151,7 +151,7 @@ RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g
 // btVehicleRL::rayCast (btVehicleRL.cpp:118-212)
 template <int MAXC>
 RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, int i,
-                          float solver_dt, gjk::Simplex& sx) {
+                          float solver_dt, gjk::Simplex& sx, bool with_mesh) {
     Car& c = A.car[ci];
     wheel_update_ws(c, g, i);
     Wheel& w = c.wheel[i];
@@ -164,7 +164,7 @@ RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGe
     w.contact_point = target;
     w.ground = -1;
     RayHit hit;
-    ray_world(mv, A, g, bg, ci, source, target, hit, sx);
+    ray_world(mv, A, g, bg, ci, source, target, hit, sx, with_mesh);
     if (hit.object >= 0) {
         float s = 1.0f - hit.fraction;
         w.contact_point = V3{s * source.x + hit.fraction * target.x, s * source.y + hit.fraction * target.y,
@@ -653,11 +653,11 @@ RSX_HD void car_update_auto_roll(Car& c, const CarGeom& g, float dt, int nwc) {
 // Car::_PreTickUpdate for a car that is not demoed (demo handling is done by the caller).
 template <int MAXC>
 RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, float dt,
-                         float solver_dt, gjk::Simplex& sx) {
+                         float solver_dt, gjk::Simplex& sx, bool rays_with_mesh = true) {
     Car& c = A.car[ci];
     // updateVehicleFirst
     for (int i = 0; i < 4; i++) wheel_update_transform(c, g, i);
-    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx);
+    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx, rays_with_mesh);
     vehicle_friction_impulses(A, c, g, dt, bg);
 
     bool jump_pressed = c.controls.jump && !c.last_controls.jump;

@@ -21,6 +21,7 @@
 #pragma once
 
 #include "rsx_car.h"
+#include "rsx_sdf.h"
 
 namespace rsx {
 
@@ -42,6 +43,8 @@ struct TickCtx {
     const PadGeom* pads;
     float dt;
     gjk::Simplex* sx;  // this lane's Voronoi simplex working set (LDS in the kernels)
+    sdf::Cull cull;    // learned broadphase cull (cull.enabled == 0: exact only)
+    uint16_t* lds_act; // the wavefront's LDS activation buffers for the cull's network
 };
 
 // --- manifold bookkeeping ----------------------------------------------------

@@ -17,6 +17,7 @@
 #include "../../include/rsx.h"
 #include "rsx_host.h"
 #include "rsx_env.h"
+#include "rsx_internal.h"
 #include "rsx_soa.h"
 
 using namespace rsx;
@@ -29,6 +30,12 @@ rsx_status fail(rsx_status s, const std::string& msg) {
     g_last_error = msg;
     return s;
 }
+
+}  // namespace
+
+int32_t rsx::record_failure(int32_t status, const char* msg) { return fail(status, msg); }
+
+namespace {
 
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
@@ -85,7 +92,14 @@ struct rsx_engine {
     int64_t obs_floats = 0;
     std::vector<int64_t> car_obs_off;  // num_cars
     std::vector<int32_t> car_obs_len;  // num_cars
+    sdf::Cull cull{};                  // learned broadphase cull (disabled: exact collision)
 };
+
+namespace rsx {
+const HostWorld& engine_host_world(const rsx_engine* e) { return e->hw; }
+int engine_device(const rsx_engine* e) { return e->device; }
+void engine_set_cull(rsx_engine* e, const sdf::Cull& c) { e->cull = c; }
+}  // namespace rsx
 
 namespace {
 
@@ -111,11 +125,24 @@ __host__ __device__ void reset_column(const Col& col, const CarGeom& cg) {
 // arena).  Their LDS arrays are sized by it.
 constexpr int kArenaBlock = 64;
 
+// LDS of an arena workgroup: the lanes' GJK / convex-cast simplices (read and
+// written many times per query, so kept out of scratch) share the space with
+// the learned cull's activation buffers, which are only live while the cull's
+// network runs at the top of a tick.
+constexpr size_t kArenaLds = sizeof(gjk::Simplex) * kArenaBlock > (size_t)sdf::kLdsActBytes
+                                 ? sizeof(gjk::Simplex) * kArenaBlock
+                                 : (size_t)sdf::kLdsActBytes;
+
+// Lanes past the last arena of the bucket run a copy of the last arena (the
+// cull's network needs every lane of the wavefront) and store nothing.
+
 template <int MAXC>
 __global__ void __launch_bounds__(kArenaBlock) k_step(uint32_t* words, int n, const rsx_controls* ctrl, int ticks,
-                                              CarGeom cg, BallGeom bg, MeshView mv, const PadGeom* pads) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
+                                              CarGeom cg, BallGeom bg, MeshView mv, const PadGeom* pads,
+                                              sdf::Cull cull) {
+    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = a0 < n;
+    const int a = live ? a0 : n - 1;
     Col col{words + a, n};
     Arena<MAXC> A;
     unpack_arena<MAXC>(col, A, cg);
@@ -130,16 +157,18 @@ __global__ void __launch_bounds__(kArenaBlock) k_step(uint32_t* words, int n, co
     T.mv = mv;
     T.pads = pads;
     T.dt = 1.f / kTickRate;
-    // the GJK / convex-cast simplex is read and written many times per query:
-    // keep it in LDS rather than in the lane's scratch
-    __shared__ gjk::Simplex sx_lds[kArenaBlock];
-    T.sx = &sx_lds[threadIdx.x];
+    __shared__ __align__(16) uint8_t lds[kArenaLds];
+    T.sx = (gjk::Simplex*)lds + threadIdx.x;
+    T.lds_act = (uint16_t*)lds;
+    T.cull = cull;
     TickScratch<MAXC> W;
+    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    if (!live) return;
     pack_arena<MAXC>(col, A);
     col.set(Layout<MAXC>::kArenaBase + kArFlags, flags);
 }
@@ -162,10 +191,11 @@ __global__ void __launch_bounds__(kArenaBlock) k_env_step(uint32_t* words, int n
                                                  const rsx_controls* lut, int lut_n, int ticks, CarGeom cg,
                                                  BallGeom bg, MeshView mv, const PadGeom* pads, EnvConfig E,
                                                  float* obs, const int64_t* obs_off, float* rew, uint8_t* term,
-                                                 const int* global) {
+                                                 const int* global, sdf::Cull cull) {
     using L = Layout<MAXC>;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
+    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = a0 < n;
+    const int a = live ? a0 : n - 1;
     Col col{words + a, n};
     Arena<MAXC> A;
     unpack_arena<MAXC>(col, A, cg);
@@ -192,16 +222,18 @@ __global__ void __launch_bounds__(kArenaBlock) k_env_step(uint32_t* words, int n
     T.mv = mv;
     T.pads = pads;
     T.dt = 1.f / kTickRate;
-    // the GJK / convex-cast simplex is read and written many times per query:
-    // keep it in LDS rather than in the lane's scratch
-    __shared__ gjk::Simplex sx_lds[kArenaBlock];
-    T.sx = &sx_lds[threadIdx.x];
+    __shared__ __align__(16) uint8_t lds[kArenaLds];
+    T.sx = (gjk::Simplex*)lds + threadIdx.x;
+    T.lds_act = (uint16_t*)lds;
+    T.cull = cull;
     TickScratch<MAXC> W;
+    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     uint32_t flags = col.get(L::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    if (!live) return;
     const bool goal = ball_scored_now(A.ball);
     bool touched[MAXC];
     bool any_touch = false;
@@ -916,7 +948,7 @@ rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stre
 #define RSX_LAUNCH_ENV(M)                                                                                        \
     hipLaunchKernelGGL(k_env_step<M>, grid, block, 0, st, b.d_words, b.n, device_actions, b.d_car_off, e->d_lut, \
                        e->lut_n, e->tick_skip, e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->env, e->d_obs,       \
-                       b.d_obs_off, e->d_rew, e->d_term, b.d_global)
+                       b.d_obs_off, e->d_rew, e->d_term, b.d_global, e->cull)
         if (b.maxc == 2) RSX_LAUNCH_ENV(2);
         else if (b.maxc == 4) RSX_LAUNCH_ENV(4);
         else RSX_LAUNCH_ENV(6);
@@ -1003,13 +1035,13 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
         dim3 grid((b.n + bs - 1) / bs), block(bs);
         if (b.maxc == 2)
             hipLaunchKernelGGL(k_step<2>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads);
+                               e->dview, e->dw.pads, e->cull);
         else if (b.maxc == 4)
             hipLaunchKernelGGL(k_step<4>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads);
+                               e->dview, e->dw.pads, e->cull);
         else
             hipLaunchKernelGGL(k_step<6>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads);
+                               e->dview, e->dw.pads, e->cull);
         HIP_TRY(hipGetLastError());
     }
     return RSX_OK;

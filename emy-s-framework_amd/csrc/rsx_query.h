@@ -149,10 +149,12 @@ RSX_HD bool ray_triangle(V3 from, V3 to, V3 v0, V3 v1, V3 v2, float& hit_fractio
 
 // Full Bullet ray test against the static world (mesh triangles near the ray,
 // then the four planes), closest hit wins (strictly closer replaces).
-RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit) {
+// with_mesh = false leaves out the mesh triangles, for a ray the learned cull
+// has shown to be out of their reach.
+RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit, bool with_mesh = true) {
     V3 mn = vmin(from, to), mx = vmax(from, to);
     int cand[48];
-    int n = gather_tris(w, mn, mx, cand, 48);
+    int n = with_mesh ? gather_tris(w, mn, mx, cand, 48) : 0;
     float frac = hit.object >= 0 ? hit.fraction : 1.0f;
     for (int i = 0; i < n; i++) {
         V3 a, b, c;

@@ -57,6 +57,8 @@ struct TickScratch {
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
     int cand[48];
     gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
+    float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
+    int cull_age;                        // ticks since the network last ran (uniform over the wavefront)
 };
 
 template <int MAXC>
@@ -275,7 +277,7 @@ RSX_HD bool sphere_box(V3 c, float r, const Xf& box, V3 h, float margin, float b
 }
 
 template <int MAXC>
-RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
+RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, const bool* far) {
     const float dt = T.dt;
     const int nb = 1 + A.ncars;
     const CarGeom& g = T.cg;
@@ -334,7 +336,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         // gathered once; a mesh with none of them cannot add a contact, and its
         // pair would only carry an empty manifold, so it is skipped.
         int ncand = 0;
-        if (mask && active[d]) {
+        if (mask && active[d] && !far[d]) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
                 ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
@@ -673,11 +675,90 @@ RSX_HD uint32_t rng_next(uint32_t& s) {
 template <int MAXC>
 RSX_HD void car_respawn(Arena<MAXC>& A, const TickCtx& T, int ci);
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// Whether a mesh triangle lies within r of q (Bullet units): the exact check
+// the learned cull is audited against.
+__device__ inline bool mesh_within(const MeshView& w, V3 q, float r) {
+    const float inv = 1.0f / w.grid_cell;
+    const V3 mn = q - V3{r, r, r}, mx = q + V3{r, r, r};
+    const int x0 = clampi((int)floorf((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y0 = clampi((int)floorf((mn.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z0 = clampi((int)floorf((mn.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    const int x1 = clampi((int)floorf((mx.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y1 = clampi((int)floorf((mx.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z1 = clampi((int)floorf((mx.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    for (int ix = x0; ix <= x1; ix++)
+        for (int iy = y0; iy <= y1; iy++)
+            for (int iz = z0; iz <= z1; iz++) {
+                const int cell = (ix * w.gy + iy) * w.gz + iz;
+                for (int k = w.cell_start[cell]; k < w.cell_start[cell + 1]; k++) {
+                    V3 a, b, c;
+                    load_tri(w, w.cell_tris[k], a, b, c);
+                    if (sdf::point_tri_dist2(q, a, b, c) <= r * r) return true;
+                }
+            }
+    return false;
+}
+
+// The learned broadphase cull of this tick: far[0] for the ball, far[1 + i]
+// for car i.  Every cull.every ticks the network predicts each body's
+// distance to the mesh; until the next evaluation a body is far when the
+// prediction minus the margin exceeds its reach plus the distance it can
+// have moved since.  A far body skips the mesh queries (narrowphase and, for
+// a car, the mesh part of its wheel rays).  The network is asked only for
+// body slots where some lane's body has a mesh listed in its broadphase cell
+// (other bodies take the exact path, which is then nearly free).  In audit
+// mode every culled body is checked against the exact mesh distance at its
+// current position.  Every lane of the wavefront must call it.
+template <int MAXC>
+__device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, bool* far) {
+    const sdf::Cull& C = T.cull;
+    const bool eval = W.cull_age >= C.every;
+    if (eval) W.cull_age = 0;
+    unsigned long long nq = 0, nc = 0, nd = 0;
+    for (int b = 0; b <= MAXC; b++) {
+        const bool have = (b == 0) || (b - 1 < A.ncars);
+        V3 q = A.ball.xf.o;
+        if (b > 0 && have) q = car_child_xf(T.cg, A.car[b - 1].xf).o;
+        if (eval) {
+            W.cull_pred[b] = -1e30f;
+            const bool need = have && T.mv.bp_meshes[bp_cell_index(T.mv, q)] != 0;
+            const unsigned long long any = __ballot(need);
+            if (any) {
+                const float d = sdf::eval_wave(C.net, q * kBT2UU, T.lds_act);
+                if (need) W.cull_pred[b] = d;
+                nq += __popcll(any);
+            }
+        }
+        const float reach = b == 0 ? C.reach_ball_uu : C.reach_car_uu;
+        const float step = b == 0 ? C.step_ball_uu : C.step_car_uu;
+        const bool cull = have && (W.cull_pred[b] - C.margin_uu > reach + step * (float)W.cull_age);
+        far[b] = cull;
+        const bool defect = cull && C.audit && mesh_within(T.mv, q, reach * kUU2BT);
+        nc += __popcll(__ballot(cull));
+        nd += __popcll(__ballot(defect));
+    }
+    W.cull_age++;
+    if ((threadIdx.x & 63) == 0) {
+        if (nq) atomicAdd(&C.stats[0], nq);
+        if (nc) atomicAdd(&C.stats[1], nc);
+        if (nd) atomicAdd(&C.stats[2], nd);
+    }
+}
+#endif
+
 // One full tick (Arena::Step body).  Returns true if the ball is in a goal
-// after the tick (Arena::IsBallScored).
+// after the tick (Arena::IsBallScored).  With the learned cull enabled every
+// lane of the wavefront must call it (the cull's network is evaluated by the
+// whole wavefront).
 template <int MAXC>
 RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     const float dt = T.dt;
+    bool far[MAXC + 1];
+    for (int b = 0; b <= MAXC; b++) far[b] = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (T.cull.enabled) cull_far<MAXC>(A, T, W, far);
+#endif
     // ball zero-velocity sleep
     A.ball.sleeping = (len2(A.ball.vel) == 0 && len2(A.ball.ang) == 0);
     for (int i = 0; i < A.ncars; i++) {
@@ -691,11 +772,15 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
         c.active = true;
         if (c.is_demoed) {
             c.demo_respawn_timer = maxf(c.demo_respawn_timer - dt, 0);
-            if (c.demo_respawn_timer == 0) car_respawn(A, T, i);
+            if (c.demo_respawn_timer == 0) {
+                car_respawn(A, T, i);
+                far[1 + i] = false;  // moved after the cull was evaluated
+                W.cull_pred[1 + i] = -1e30f;
+            }
             c.active = false;  // body disabled for this tick
         }
         if (c.is_demoed) continue;
-        car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx);
+        car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i]);
     }
     if (A.ncars > 0) {
         for (int p = 0; p < kNumPads; p++) {
@@ -707,7 +792,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     }
     M3 rot0[MAXC];
     for (int i = 0; i < A.ncars; i++) rot0[i] = A.car[i].xf.b;
-    arena_world_step(A, T, W);
+    arena_world_step(A, T, W, far);
     for (int i = 0; i < A.ncars; i++) {
         Car& c = A.car[i];
         // A car demolished during this step skips Car::_PostTickUpdate, which

@@ -136,6 +136,8 @@ int cc_step(void* p, int ticks) {
     T.dt = 1.f / kTickRate;
     gjk::Simplex sx;
     T.sx = &sx;
+    T.cull = sdf::Cull{};
+    T.lds_act = nullptr;
     int scored = 0;
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;
@@ -225,6 +227,17 @@ extern "C" int cc_tri_info(void* w, float* out, int max_tris) {
         o[13] = (float)h.tri_mesh[t];
     }
     return n;
+}
+
+// Squared point-triangle distance of the learned cull's exact query, for n
+// (point, triangle) pairs: pts 3 floats each, tris 9 floats each.
+extern "C" void cc_point_tri_dist2(const float* pts, const float* tris, int n, float* out) {
+    for (int i = 0; i < n; i++) {
+        const float* p = pts + 3 * i;
+        const float* t = tris + 9 * i;
+        out[i] = sdf::point_tri_dist2(V3{p[0], p[1], p[2]}, V3{t[0], t[1], t[2]}, V3{t[3], t[4], t[5]},
+                                      V3{t[6], t[7], t[8]});
+    }
 }
 
 // Ray test of one convex body, as the suspension ray test does it for the

@@ -124,6 +124,7 @@ def load_cpucheck():
         ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
         ("cc_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
         ("cc_tri_info", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+        ("cc_point_tri_dist2", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
         ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ])

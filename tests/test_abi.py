@@ -11,7 +11,7 @@ import pytest
 import rsxlib as R
 
 ROOT = R.ROOT
-HEADERS = [os.path.join(ROOT, "include", f) for f in ("rsx.h",)]
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("rsx.h", "rsx_sdf.h")]
 
 
 def declared_functions():
