@@ -105,6 +105,19 @@ def cpu_baseline(seconds, threads, tick_skip=8):
                       f"physics only) on {threads} threads, {dt:.1f} s"}
 
 
+def measured_traffic(workload):
+    """HBM bytes per k_env_step<2> launch from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-scaled),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"k_env_step_1v1_65536_{workload}_states.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,7 +225,7 @@ def main():
         "cull": cull,
         "audit": audit,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload) if args.arenas == 65536 else None,
                      "kernel": "k_env_step<2>", "kernel_ms": head["kernel_ms"],
                      "algo_bytes_per_env_step": ALGO_BYTES_1V1},
         "cpu_baseline": cpu,
