@@ -32,6 +32,15 @@ HBM_PEAK_GBS = 8000.0
 ALGO_BYTES_1V1 = 2 * 1132 + 2 * 4 + 2 * 89 * 4 + 2 * 4 + 1
 
 
+def algo_bytes_per_env_step(team):
+    """The same convention for an arena of `team` cars per side: 400 B per
+    car, 36 B ball, 272 B pads and 24 B arena record read and written, and
+    per car an action in, an obs row of 51 + 19 x cars floats and a reward
+    out, plus the terminal byte."""
+    cars = 2 * team
+    return 2 * (400 * cars + 36 + 272 + 24) + cars * 4 + cars * (51 + 19 * cars) * 4 + cars * 4 + 1
+
+
 def _import_package():
     if "emy_s_framework_amd" in sys.modules:
         return sys.modules["emy_s_framework_amd"]
@@ -123,9 +132,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--arenas", type=int, default=65536)
+    ap.add_argument("--arenas", type=int, default=None, help="default 65536 (16384 for the mixed workload)")
     ap.add_argument("--tick-skip", type=int, default=8)
-    ap.add_argument("--workload", choices=("stress", "random"), default="stress")
+    ap.add_argument("--workload", choices=("stress", "random", "mixed"), default="stress")
+    ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=True,
+                    help="replay each env step from a captured hipGraph")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
+                    help="pick the launch shape with the library's autotuner first")
+    ap.add_argument("--tuning", default=None, help="fixed launch shape 'block,arenas_per_wave' (skips the autotuner)")
     ap.add_argument("--cull", choices=("auto", "on", "off"), default="auto")
     ap.add_argument("--audit-steps", type=int, default=8)
     ap.add_argument("--cull-every", type=int, default=8)
@@ -140,12 +154,21 @@ def main():
     import torch
     eng = _import_package()
     from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf
-    from emy_s_framework_amd.workloads import random_field_states, stress_states
+    from emy_s_framework_amd.workloads import mixed_team_sizes, random_field_states, stress_states
 
     torch.cuda.init()
     dev = torch.device("cuda:0")
-    batch = eng.RocketSimBatch(args.arenas, 1, tick_skip=args.tick_skip, seed=4, device=0)
-    apply_states = stress_states if args.workload == "stress" else random_field_states
+    if args.arenas is None:
+        args.arenas = 16384 if args.workload == "mixed" else 65536
+    if args.workload == "mixed":
+        teams = mixed_team_sizes(args.arenas, seed=2)
+        batch = eng.RocketSimBatch(args.arenas, teams, tick_skip=args.tick_skip, seed=2, device=0)
+        apply_states = lambda b, seed: random_field_states(b, 2)  # noqa: E731
+    else:
+        teams = [1] * args.arenas
+        batch = eng.RocketSimBatch(args.arenas, 1, tick_skip=args.tick_skip, seed=4, device=0)
+        apply_states = stress_states if args.workload == "stress" else random_field_states
+    algo_bytes = sum(algo_bytes_per_env_step(t) for t in teams)
     batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
     n_cars = batch.num_cars
     stream = torch.cuda.current_stream(dev)
@@ -161,26 +184,60 @@ def main():
         net = NeuralSdf(batch, seed=0)
         margin = float(net.load(WEIGHTS_PATH)["margin_uu"])
 
-    def run(tag, steps, warmup):
-        """Fresh workload states, warmup, then `steps` timed fused env steps."""
+    d_act = torch.empty(n_cars, dtype=torch.int32, device=dev)  # the captured graph's action buffer
+
+    def tune():
+        """Launch shape for the current configuration (exact or culled)."""
+        if args.tuning:
+            batch.SetTuning(*[int(x) for x in args.tuning.split(",")])
+            return None
+        if not args.autotune:
+            return None
         apply_states(batch, seed=4)
+        d_act.copy_(actions[0])
+        res = batch.Autotune(d_act.data_ptr(), env_steps=2)
+        return [{"block": b, "arenas_per_wave": a, "ms": ms} for b, a, ms in res]
+
+    def run(tag, steps, warmup, retune=True):
+        """Fresh workload states, warmup, then `steps` timed fused env steps
+        (with --graph each replayed from the captured hipGraph, whose action
+        buffer is refreshed by a device copy before the replay)."""
+        tuning = tune() if retune else None
+        apply_states(batch, seed=4)
+        if args.graph:
+            batch.CaptureEnvGraph(d_act.data_ptr(), 1)
+
+        def env_step(i, ev=None):
+            if args.graph:
+                d_act.copy_(actions[i % len(actions)])
+                if ev:
+                    ev[0].record(stream)
+                batch.LaunchEnvGraph(sp)
+            else:
+                if ev:
+                    ev[0].record(stream)
+                batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+            if ev:
+                ev[1].record(stream)
+
         torch.cuda.synchronize()
         for i in range(warmup):
-            batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+            env_step(i)
         torch.cuda.synchronize()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(steps):
-            evs[i][0].record(stream)
-            batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
-            evs[i][1].record(stream)
+            env_step(i, evs[i])
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         kern = sum(a.elapsed_time(b) for a, b in evs) / steps
         print(f"# {tag}: {steps} steps, {wall * 1e3 / steps:.2f} ms/step (kernel {kern:.2f} ms)", file=sys.stderr,
               flush=True)
-        return {"value": args.arenas * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern}
+        shape = batch.GetTuning()
+        return {"value": args.arenas * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern,
+                "agent_steps_per_s": n_cars * steps / wall,
+                "launch_shape": {"block": shape[0], "arenas_per_wave": shape[1]}, "autotune": tuning}
 
     exact = run("exact collision", args.steps, args.warmup)
     cull = audit = None
@@ -189,19 +246,23 @@ def main():
         cull = run("learned cull", args.steps, args.warmup)
         net.attach(margin, every=args.cull_every, audit=True)
         net.reset_stats()
-        run("learned cull, audited", args.audit_steps, 0)
+        run("learned cull, audited", args.audit_steps, 0, retune=False)
         audit = net.stats()
-        body_ticks = args.audit_steps * args.tick_skip * args.arenas * 3
+        body_ticks = args.audit_steps * args.tick_skip * (args.arenas + n_cars)
         audit.update({"steps": args.audit_steps, "margin_uu": margin, "every_ticks": args.cull_every,
                       "culled_fraction": audit["culled"] / body_ticks})
         net.detach()
     head = cull if cull is not None else exact
-    achieved = ALGO_BYTES_1V1 * args.arenas / (head["kernel_ms"] * 1e-3) / 1e9
+    achieved = algo_bytes / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds, max(1, min(16, os.cpu_count() or 1)))
-    wl = ("contact-heavy states (seed 4: 30% ball near corners/ramps/goal mouths, 20% cars on walls or "
-          "ceiling, 10% overlapping car spawns)" if args.workload == "stress" else "random field states (seed 4)")
+    wl = {"stress": "contact-heavy states (seed 4: 30% ball near corners/ramps/goal mouths, 20% cars on walls or "
+                    "ceiling, 10% overlapping car spawns)",
+          "random": "random field states (seed 4)",
+          "mixed": "random field states (seed 2), team sizes 1v1 50% / 2v2 30% / 3v3 20% shuffled (seed 2), "
+                   "one launch over all team sizes"}[args.workload]
+    kinds = "1v1" if args.workload != "mixed" else "mixed 1v1/2v2/3v3"
     out = {
         "metric": "env-steps/sec @65536 1v1 arenas (neural-SDF cull vs exact), % HBM roofline",
         "value": head["value"],
@@ -215,19 +276,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.arenas} soccar 1v1 arenas, {wl}, uniform random discrete actions, "
+        "config": {"workload": f"{args.arenas} soccar {kinds} arenas, {wl}, uniform random discrete actions, "
                                f"tick-skip {args.tick_skip}, fused env step (physics + DefaultObs + 4 stub rewards "
-                               f"+ terminals + auto-reset)",
-                   "arenas": args.arenas, "tick_skip": args.tick_skip,
+                               f"+ terminals + auto-reset)" + (", replayed from a hipGraph" if args.graph else ""),
+                   "arenas": args.arenas, "agents": n_cars, "tick_skip": args.tick_skip,
                    "cull": (f"learned SDF cull every {args.cull_every} ticks, margin {margin:.1f} uu, exact "
                             f"fallback" if cull is not None else "none (exact collision)")},
         "exact": exact,
         "cull": cull,
         "audit": audit,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload) if args.arenas == 65536 else None,
-                     "kernel": "k_env_step<2>", "kernel_ms": head["kernel_ms"],
-                     "algo_bytes_per_env_step": ALGO_BYTES_1V1},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload) if args.arenas == 65536 else None,
+                     "kernel": "k_env_mixed" if args.workload == "mixed" else "k_env_step<2>",
+                     "kernel_ms": head["kernel_ms"], "algo_bytes_per_launch": algo_bytes,
+                     "algo_bytes_per_env_step_1v1": ALGO_BYTES_1V1},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -8,7 +8,7 @@ import ctypes as C
 import os
 
 from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, EnvConfig, NUM_PADS,
-                      PadStates, CARS_BLUE_FIRST)
+                      PadStates, CARS_BLUE_FIRST, Tuning, TuneResult)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
@@ -56,6 +56,11 @@ _SIGS = [
     ("rsx_copy_terminals", _I, [_VP, C.POINTER(C.c_uint8), _I]),
     ("rsx_step_ticks", _I, [_VP, _I, _VP]),
     ("rsx_synchronize", _I, [_VP]),
+    ("rsx_set_tuning", _I, [_VP, C.POINTER(Tuning)]),
+    ("rsx_get_tuning", _I, [_VP, C.POINTER(Tuning)]),
+    ("rsx_autotune", _I, [_VP, _VP, _I, C.POINTER(TuneResult), _I, C.POINTER(_I)]),
+    ("rsx_capture_env_graph", _I, [_VP, _VP, _I]),
+    ("rsx_launch_env_graph", _I, [_VP, _VP]),
     # learned broadphase cull (include/rsx_sdf.h)
     ("rsx_sdf_num_params", _I, []),
     ("rsx_sdf_create", _I, [_VP, C.c_uint64, C.POINTER(_VP)]),
@@ -300,3 +305,28 @@ class RocketSimBatch:
 
     def Synchronize(self):
         _check(self._lib.rsx_synchronize(self._h))
+
+    def SetTuning(self, block_lanes=64, arenas_per_wave=64):
+        """Launch shape of the arena kernels (results do not depend on it)."""
+        _check(self._lib.rsx_set_tuning(self._h, C.byref(Tuning(block_lanes, arenas_per_wave))))
+
+    def GetTuning(self):
+        t = Tuning()
+        _check(self._lib.rsx_get_tuning(self._h, C.byref(t)))
+        return t.block_lanes, t.arenas_per_wave
+
+    def Autotune(self, device_actions_ptr, env_steps=3):
+        """Times every launch shape on the current arenas, keeps the fastest and
+        restores the arenas; returns [(block_lanes, arenas_per_wave, ms or None)]."""
+        res = (TuneResult * 16)()
+        n = _I()
+        _check(self._lib.rsx_autotune(self._h, device_actions_ptr, env_steps, res, 16, C.byref(n)))
+        return [(r.tuning.block_lanes, r.tuning.arenas_per_wave, r.ms_per_env_step if r.feasible else None)
+                for r in res[:n.value]]
+
+    def CaptureEnvGraph(self, device_actions_ptr, env_steps=1):
+        """Records env_steps StepEnv calls on device_actions_ptr into a hipGraph."""
+        _check(self._lib.rsx_capture_env_graph(self._h, device_actions_ptr, env_steps))
+
+    def LaunchEnvGraph(self, stream=None):
+        _check(self._lib.rsx_launch_env_graph(self._h, stream))

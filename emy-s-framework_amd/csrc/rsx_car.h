@@ -97,15 +97,19 @@ RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
 //
 // Every car whose wheels are cast is live, so its own AABB marks the
 // suspension grid cells around its hardpoints and the grid always defers to
-// the full ray test (SuspensionCollisionGrid.cpp:131-133); and a dynamic body
-// is registered in the 3x3x3 broadphase cells around its AABB minimum, which
-// covers every body a suspension-length ray from the origin's cell can reach.
+// the full ray test (SuspensionCollisionGrid.cpp:131-133).  That test
+// (btRSBroadphase::rayTest, btRSBroadphase.cpp:326-336) visits the bodies
+// listed in the cell the ray starts in: a dynamic body is listed in the 3x3x3
+// cells around the cell of its AABB minimum as of its last AABB update, which
+// covers every body the ray can reach unless the body's state was set since
+// the last world step (Car::bp_reg).
 template <int MAXC>
 RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int self,
                       V3 from, V3 to, RayHit& hit, gjk::Simplex& sx, bool with_mesh) {
     hit.object = -1;
     hit.fraction = 1.0f;
     ray_static_world(mv, from, to, hit, with_mesh);
+    const int from_cell = bp_cell_index(mv, from);  // btRSBroadphase::rayTest: the start cell's list
     const V3 seg = to - from;
     const float seg2 = dot(seg, seg);
     // A cast only reports a hit where the ray comes within its stopping
@@ -135,10 +139,10 @@ RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g
         s.kind = 1;
         s.h = V3{bg.radius, 0, 0};
         s.xf = A.ball.xf;
-        try_body(s, 1);
+        if (bp_listed(mv, A.ball.bp_reg, from_cell)) try_body(s, 1);
     }
     for (int k = 0; k < A.ncars; k++) {
-        if (k == self) continue;
+        if (k == self || !bp_listed(mv, A.car[k].bp_reg, from_cell)) continue;
         gjk::RayShape s;
         s.kind = 0;
         s.h = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};

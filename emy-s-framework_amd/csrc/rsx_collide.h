@@ -44,7 +44,8 @@ struct TickCtx {
     float dt;
     gjk::Simplex* sx;  // this lane's Voronoi simplex working set (LDS in the kernels)
     sdf::Cull cull;    // learned broadphase cull (cull.enabled == 0: exact only)
-    uint16_t* lds_act; // the wavefront's LDS activation buffers for the cull's network
+    uint16_t* lds_act; // the wavefront's LDS activation buffer for the cull's network
+    bool own;          // this lane stores its arena (false: it mirrors another lane, or pads the grid)
 };
 
 // --- manifold bookkeeping ----------------------------------------------------
@@ -219,16 +220,6 @@ RSX_HD void plane_shape_aabb(int p, V3& mn, V3& mx) {
     if (p == 2) { mn.x = o.x + (0 - off); mx.x = o.x + (0 + off); mn.x = -L; }
     if (p == 3) { mn.x = o.x + (0 - off); mx.x = o.x + (0 + off); mx.x = L; }
 }
-
-RSX_HD int bp_cell_index(const MeshView& w, V3 pos) {
-    V3 f = (pos - V3{kBpMinX, kBpMinY, kBpMinZ}) / kBpCellSize;
-    int i = (int)f.x, j = (int)f.y, k = (int)f.z;
-    i = i < 0 ? 0 : (i > w.bp_nx - 1 ? w.bp_nx - 1 : i);
-    j = j < 0 ? 0 : (j > w.bp_ny - 1 ? w.bp_ny - 1 : j);
-    k = k < 0 ? 0 : (k > w.bp_nz - 1 ? w.bp_nz - 1 : k);
-    return i * w.bp_ny * w.bp_nz + j * w.bp_nz + k;
-}
-
 
 // btAdjustInternalEdgeContacts (btInternalEdgeUtility.cpp:330-690) with
 // normalAdjustFlags = 0; the mesh transform is the identity.

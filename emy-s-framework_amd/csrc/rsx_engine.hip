@@ -1,11 +1,7 @@
-// rsx_engine.hip — gfx950 kernels and the C-ABI of include/rsx.h.
-//
-// Work mapping: one lane owns one arena for a whole launch.  A launch loads
-// the arena's words from the field-major SoA buffer (coalesced across the 64
-// lanes of a wavefront), runs the requested ticks entirely in registers and
-// private memory, and stores the words back.  Arenas are grouped into three
-// buckets by team size (1v1, 2v2, 3v3) and each bucket is stepped by its own
-// instantiation of the kernel, so no wavefront mixes car counts.
+// rsx_engine.hip — the C-ABI of include/rsx.h: engine set-up, state access,
+// resets, action expansion and obs kernels, and the launches of the arena
+// kernels (rsx_kern.h), with their launch-shape autotuner and hipGraph
+// capture of the env step.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -18,6 +14,7 @@
 #include "rsx_host.h"
 #include "rsx_env.h"
 #include "rsx_internal.h"
+#include "rsx_kern.h"
 #include "rsx_soa.h"
 
 using namespace rsx;
@@ -93,17 +90,25 @@ struct rsx_engine {
     std::vector<int64_t> car_obs_off;  // num_cars
     std::vector<int32_t> car_obs_len;  // num_cars
     sdf::Cull cull{};                  // learned broadphase cull (disabled: exact collision)
+    LaunchShape shape;                 // launch shape of the arena kernels
+    uint64_t gen = 1;                  // bumped by every change that alters the env step's launches
+    size_t lds_per_block_max = 65536;  // device limit of a workgroup's LDS
+    // captured env steps (rsx_capture_env_graph)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_gen = 0;
 };
 
 namespace rsx {
 const HostWorld& engine_host_world(const rsx_engine* e) { return e->hw; }
 int engine_device(const rsx_engine* e) { return e->device; }
-void engine_set_cull(rsx_engine* e, const sdf::Cull& c) { e->cull = c; }
+void engine_set_cull(rsx_engine* e, const sdf::Cull& c) {
+    e->cull = c;
+    e->gen++;
+}
 }  // namespace rsx
 
 namespace {
-
-constexpr uint32_t kFlagGoal = 1u, kFlagGoalOrange = 2u;
 
 // Kickoff reset of one arena column in place: keeps car ids, teams and the
 // carried wheel values (Car::SetState does not touch them), advances the
@@ -121,58 +126,6 @@ __host__ __device__ void reset_column(const Col& col, const CarGeom& cg) {
     col.set(L::kArenaBase + kArEpisode, col.get(L::kArenaBase + kArEpisode) + 1u);
 }
 
-// Lanes per workgroup of the arena kernels (one wavefront; one lane per
-// arena).  Their LDS arrays are sized by it.
-constexpr int kArenaBlock = 64;
-
-// LDS of an arena workgroup: the lanes' GJK / convex-cast simplices (read and
-// written many times per query, so kept out of scratch) share the space with
-// the learned cull's activation buffers, which are only live while the cull's
-// network runs at the top of a tick.
-constexpr size_t kArenaLds = sizeof(gjk::Simplex) * kArenaBlock > (size_t)sdf::kLdsActBytes
-                                 ? sizeof(gjk::Simplex) * kArenaBlock
-                                 : (size_t)sdf::kLdsActBytes;
-
-// Lanes past the last arena of the bucket run a copy of the last arena (the
-// cull's network needs every lane of the wavefront) and store nothing.
-
-template <int MAXC>
-__global__ void __launch_bounds__(kArenaBlock) k_step(uint32_t* words, int n, const rsx_controls* ctrl, int ticks,
-                                              CarGeom cg, BallGeom bg, MeshView mv, const PadGeom* pads,
-                                              sdf::Cull cull) {
-    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = a0 < n;
-    const int a = live ? a0 : n - 1;
-    Col col{words + a, n};
-    Arena<MAXC> A;
-    unpack_arena<MAXC>(col, A, cg);
-    for (int i = 0; i < A.ncars; i++) {
-        const rsx_controls& c = ctrl[(int64_t)a * MAXC + i];
-        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
-                                     c.handbrake != 0};
-    }
-    TickCtx T;
-    T.cg = cg;
-    T.bg = bg;
-    T.mv = mv;
-    T.pads = pads;
-    T.dt = 1.f / kTickRate;
-    __shared__ __align__(16) uint8_t lds[kArenaLds];
-    T.sx = (gjk::Simplex*)lds + threadIdx.x;
-    T.lds_act = (uint16_t*)lds;
-    T.cull = cull;
-    TickScratch<MAXC> W;
-    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
-    uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
-    for (int t = 0; t < ticks; t++) {
-        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
-            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
-    }
-    if (!live) return;
-    pack_arena<MAXC>(col, A);
-    col.set(Layout<MAXC>::kArenaBase + kArFlags, flags);
-}
-
 // Kickoff reset of the arenas whose mask byte is non-zero (all when mask is null).
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint8_t* mask, CarGeom cg) {
@@ -181,87 +134,6 @@ __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint
     if (mask && !mask[a]) return;
     Col col{words + a, n};
     reset_column<MAXC>(col, cg);
-}
-
-// Fused env step: action indices -> controls, `ticks` physics ticks, then the
-// stub rewards, the terminal conditions, the kickoff auto-reset of terminal
-// arenas and the DefaultObs rows, all in the lane that owns the arena.
-template <int MAXC>
-__global__ void __launch_bounds__(kArenaBlock) k_env_step(uint32_t* words, int n, const int32_t* act, const int* car_off,
-                                                 const rsx_controls* lut, int lut_n, int ticks, CarGeom cg,
-                                                 BallGeom bg, MeshView mv, const PadGeom* pads, EnvConfig E,
-                                                 float* obs, const int64_t* obs_off, float* rew, uint8_t* term,
-                                                 const int* global, sdf::Cull cull) {
-    using L = Layout<MAXC>;
-    const int a0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = a0 < n;
-    const int a = live ? a0 : n - 1;
-    Col col{words + a, n};
-    Arena<MAXC> A;
-    unpack_arena<MAXC>(col, A, cg);
-    float prev[MAXC][kActionFloats];
-    const int c0 = car_off[a];
-    for (int i = 0; i < A.ncars; i++) {
-        int k = act[c0 + i];
-        k = k < 0 ? 0 : (k >= lut_n ? lut_n - 1 : k);
-        const rsx_controls c = lut[k];
-        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
-                                     c.handbrake != 0};
-        prev[i][0] = c.throttle;
-        prev[i][1] = c.steer;
-        prev[i][2] = c.pitch;
-        prev[i][3] = c.yaw;
-        prev[i][4] = c.roll;
-        prev[i][5] = c.jump ? 1.f : 0.f;
-        prev[i][6] = c.boost ? 1.f : 0.f;
-        prev[i][7] = c.handbrake ? 1.f : 0.f;
-    }
-    TickCtx T;
-    T.cg = cg;
-    T.bg = bg;
-    T.mv = mv;
-    T.pads = pads;
-    T.dt = 1.f / kTickRate;
-    __shared__ __align__(16) uint8_t lds[kArenaLds];
-    T.sx = (gjk::Simplex*)lds + threadIdx.x;
-    T.lds_act = (uint16_t*)lds;
-    T.cull = cull;
-    TickScratch<MAXC> W;
-    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
-    uint32_t flags = col.get(L::kArenaBase + kArFlags);
-    for (int t = 0; t < ticks; t++) {
-        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
-            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
-    }
-    if (!live) return;
-    const bool goal = ball_scored_now(A.ball);
-    bool touched[MAXC];
-    bool any_touch = false;
-    for (int i = 0; i < A.ncars; i++) {
-        touched[i] = touched_in_step(A.car[i], A.tick, ticks);
-        any_touch |= touched[i];
-    }
-    float no_touch = col.getf(L::kArenaBase + kArNoTouch);
-    const uint8_t tt = step_terminal(goal, any_touch, (float)ticks * (1.f / kTickRate), no_touch, E);
-    for (int i = 0; i < A.ncars; i++) rew[c0 + i] = stub_reward<MAXC>(A, i, goal, touched[i], E);
-    uint32_t episode = col.get(L::kArenaBase + kArEpisode);
-    if (tt != kNotTerminal && E.auto_reset) {
-        arena_reset_kickoff<MAXC>(A, cg);
-        A.tick = 0;
-        flags = 0;
-        no_touch = 0.f;
-        episode++;
-        for (int i = 0; i < A.ncars; i++)
-            for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
-    }
-    const int row = obs_row_len(A.ncars);
-    float* o = obs + obs_off[a];
-    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
-    pack_arena<MAXC>(col, A);
-    col.set(L::kArenaBase + kArFlags, flags);
-    col.setf(L::kArenaBase + kArNoTouch, no_touch);
-    col.set(L::kArenaBase + kArEpisode, episode);
-    term[global[a]] = tt;
 }
 
 // DefaultObs rows of the current state with zero previous actions (the
@@ -305,16 +177,26 @@ int bucket_of_team(int t) { return t - 1; }
 
 // Host-side pack/unpack of one arena through a contiguous staging column.
 template <int MAXC>
-void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, int team, uint32_t seed, int slot_base_id,
-                      int car_order) {
+void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, const BallGeom& bg, const MeshView& mv, int team,
+                      uint32_t seed, int slot_base_id, int car_order) {
     Arena<MAXC> A;
     memset(&A, 0, sizeof(A));
+    // broadphase registrations where Bullet adds the bodies: cars at the
+    // identity transform (Car.cpp:211-228), the ball resting at the origin
+    // (Ball.cpp:90-91)
+    V3 mn, mx;
+    car_shape_aabb(g, xf_identity(), mn, mx);
+    const int car_reg = bp_cell_index(mv, mn);
+    ball_shape_aabb(bg, V3{0, 0, bg.radius}, mn, mx);
+    A.ball.bp_reg = bp_cell_index(mv, mn);
     A.ncars = 2 * team;
     A.rng = seed ? seed : 0x9E3779B9u;
     A.tick = 0;
     for (int i = 0; i < A.ncars; i++) {
         A.car[i].team = car_order == RSX_CARS_ALTERNATING ? (i & 1) : (i < team ? 0 : 1);
         A.car[i].id = (uint32_t)(slot_base_id + i + 1);
+        A.car[i].active = true;
+        A.car[i].bp_reg = car_reg;
     }
     for (int i = 0; i < A.ncars; i++) car_set_spawn<MAXC>(A.car[i], g, 0, 0, kCarSpawnRestZ, 0, kBoostSpawnAmount);
     colbuf.assign(Layout<MAXC>::kWords, 0);
@@ -323,6 +205,60 @@ void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, int team,
     c.set(Layout<MAXC>::kArenaBase + kArNcars, (uint32_t)A.ncars);
     reset_column<MAXC>(c, g);
     c.set(Layout<MAXC>::kArenaBase + kArEpisode, 0u);
+}
+
+#define RSX_TRY(expr)                    \
+    do {                                 \
+        rsx_status _s = (expr);          \
+        if (_s != RSX_OK) return _s;     \
+    } while (0)
+
+BucketArgs bucket_args(const Bucket& b) {
+    BucketArgs a;
+    a.words = b.d_words;
+    a.n = b.n;
+    a.ctrl = b.d_ctrl;
+    a.car_off = b.d_car_off;
+    a.obs_off = b.d_obs_off;
+    a.global = b.d_global;
+    return a;
+}
+
+WorldArgs world_args(const rsx_engine* e) { return WorldArgs{e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->cull}; }
+
+// Checks that the shape's LDS fits a workgroup and raises the kernels'
+// dynamic-LDS limit when it needs more than 64 KiB.
+rsx_status prepare_shape(rsx_engine* e, const LaunchShape& s) {
+    const size_t lds = arena_lds_per_wave(e->cull.enabled != 0) * (s.block / 64);
+    if (lds > e->lds_per_block_max)
+        return fail(RSX_INVALID_ARG, "launch shape needs " + std::to_string(lds) + " bytes of LDS per workgroup; the "
+                                         "device allows " + std::to_string(e->lds_per_block_max));
+    HIP_TRY(prepare_arena_kernels(lds));
+    return RSX_OK;
+}
+
+// One fused env step of every arena: one launch (the mixed kernel when team
+// sizes differ).
+hipError_t env_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, hipStream_t st) {
+    const WorldArgs w = world_args(e);
+    EnvArgs v;
+    v.act = act;
+    v.lut = e->d_lut;
+    v.lut_n = e->lut_n;
+    v.ticks = e->tick_skip;
+    v.E = e->env;
+    v.obs = e->d_obs;
+    v.rew = e->d_rew;
+    v.term = e->d_term;
+    int nonempty = 0;
+    for (const Bucket& b : e->bucket) nonempty += b.n > 0;
+    if (nonempty > 1) {
+        const BucketArgs b[3] = {bucket_args(e->bucket[0]), bucket_args(e->bucket[1]), bucket_args(e->bucket[2])};
+        return launch_env_mixed(b, w, v, s, st);
+    }
+    for (const Bucket& b : e->bucket)
+        if (b.n) return launch_env(b.maxc, bucket_args(b), w, v, s, st);
+    return hipSuccess;
 }
 
 rsx_status column_get(rsx_engine* e, int arena, std::vector<uint32_t>& out, Bucket*& b, int& local) {
@@ -381,6 +317,12 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return fail(RSX_HIP_ERROR, "no HIP device is available");
     if (e->device < 0 || e->device >= nd) return fail(RSX_INVALID_ARG, "device ordinal out of range");
     HIP_TRY(hipSetDevice(e->device));
+    {
+        int a = 0, b = 0;
+        HIP_TRY(hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
+        if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, e->device) != hipSuccess) b = 0;
+        e->lds_per_block_max = (size_t)std::max(a, b);
+    }
     std::string err;
     if (!load_world(cfg->mesh_dir, e->hw, err)) return fail(RSX_MESH_LOAD_FAILED, err);
     rsx_status s;
@@ -486,11 +428,12 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         HIP_TRY(hipMemset(b.d_ctrl, 0, sizeof(rsx_controls) * (size_t)b.maxc * b.n));
         // initial kickoff states, built on the host and uploaded field-major
         std::vector<uint32_t> soa((size_t)b.words * b.n), col;
+        const MeshView hv = e->hw.view();
         for (int l = 0; l < b.n; l++) {
             uint32_t seed = cfg->seed * 2654435761u + (uint32_t)b.global[l] * 0x9E3779B9u + 1u;
-            if (b.maxc == 2) host_reset_arena<2>(col, e->hw.cg, 1, seed, 0, cfg->car_order);
-            else if (b.maxc == 4) host_reset_arena<4>(col, e->hw.cg, 2, seed, 0, cfg->car_order);
-            else host_reset_arena<6>(col, e->hw.cg, 3, seed, 0, cfg->car_order);
+            if (b.maxc == 2) host_reset_arena<2>(col, e->hw.cg, e->hw.bg, hv, 1, seed, 0, cfg->car_order);
+            else if (b.maxc == 4) host_reset_arena<4>(col, e->hw.cg, e->hw.bg, hv, 2, seed, 0, cfg->car_order);
+            else host_reset_arena<6>(col, e->hw.cg, e->hw.bg, hv, 3, seed, 0, cfg->car_order);
             for (int w = 0; w < b.words; w++) soa[(size_t)w * b.n + l] = col[w];
         }
         HIP_TRY(hipMemcpy(b.d_words, soa.data(), soa.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -517,6 +460,8 @@ rsx_status rsx_destroy(rsx_engine* e) {
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
     for (void* p : ptrs) (void)hipFree(p);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
     delete e;
     return RSX_OK;
 }
@@ -720,6 +665,7 @@ rsx_status rsx_upload_action_lut(rsx_engine* e, const rsx_controls* lut, int32_t
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipMemcpy(e->d_lut, lut, sizeof(rsx_controls) * count, hipMemcpyHostToDevice));
     e->lut_n = count;
+    e->gen++;
     return RSX_OK;
 }
 
@@ -919,6 +865,7 @@ rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c) {
     E.no_touch_timeout = c->no_touch_timeout_s;
     E.goal_terminal = c->goal_terminal != 0;
     E.auto_reset = c->auto_reset != 0;
+    e->gen++;
     return RSX_OK;
 }
 
@@ -940,21 +887,8 @@ rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stre
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (!device_actions) return fail(RSX_INVALID_ARG, "null actions");
     HIP_TRY(hipSetDevice(e->device));
-    hipStream_t st = (hipStream_t)stream;
-    for (int k = 0; k < 3; k++) {
-        Bucket& b = e->bucket[k];
-        if (!b.n) continue;
-        dim3 grid((b.n + kArenaBlock - 1) / kArenaBlock), block(kArenaBlock);
-#define RSX_LAUNCH_ENV(M)                                                                                        \
-    hipLaunchKernelGGL(k_env_step<M>, grid, block, 0, st, b.d_words, b.n, device_actions, b.d_car_off, e->d_lut, \
-                       e->lut_n, e->tick_skip, e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->env, e->d_obs,       \
-                       b.d_obs_off, e->d_rew, e->d_term, b.d_global, e->cull)
-        if (b.maxc == 2) RSX_LAUNCH_ENV(2);
-        else if (b.maxc == 4) RSX_LAUNCH_ENV(4);
-        else RSX_LAUNCH_ENV(6);
-#undef RSX_LAUNCH_ENV
-        HIP_TRY(hipGetLastError());
-    }
+    RSX_TRY(prepare_shape(e, e->shape));
+    HIP_TRY(env_launch(e, device_actions, e->shape, (hipStream_t)stream));
     return RSX_OK;
 }
 
@@ -1027,23 +961,153 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (ticks < 0) return fail(RSX_INVALID_ARG, "ticks must be >= 0");
     HIP_TRY(hipSetDevice(e->device));
-    hipStream_t st = (hipStream_t)stream;
-    const int bs = kArenaBlock;
+    RSX_TRY(prepare_shape(e, e->shape));
+    const WorldArgs w = world_args(e);
     for (int k = 0; k < 3; k++) {
-        Bucket& b = e->bucket[k];
-        if (!b.n) continue;
-        dim3 grid((b.n + bs - 1) / bs), block(bs);
-        if (b.maxc == 2)
-            hipLaunchKernelGGL(k_step<2>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads, e->cull);
-        else if (b.maxc == 4)
-            hipLaunchKernelGGL(k_step<4>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads, e->cull);
-        else
-            hipLaunchKernelGGL(k_step<6>, grid, block, 0, st, b.d_words, b.n, b.d_ctrl, ticks, e->hw.cg, e->hw.bg,
-                               e->dview, e->dw.pads, e->cull);
-        HIP_TRY(hipGetLastError());
+        const Bucket& b = e->bucket[k];
+        if (b.n) HIP_TRY(launch_ticks(b.maxc, bucket_args(b), w, ticks, e->shape, (hipStream_t)stream));
     }
+    return RSX_OK;
+}
+
+rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!t) return fail(RSX_INVALID_ARG, "null tuning");
+    const LaunchShape s{t->block_lanes, t->arenas_per_wave};
+    if (!shape_valid(s))
+        return fail(RSX_INVALID_ARG, "block_lanes must be 64, 128 or 256 and arenas_per_wave 64, 32, 16 or 8");
+    HIP_TRY(hipSetDevice(e->device));
+    RSX_TRY(prepare_shape(e, s));
+    e->shape = s;
+    e->gen++;
+    return RSX_OK;
+}
+
+rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!t) return fail(RSX_INVALID_ARG, "null tuning");
+    t->block_lanes = e->shape.block;
+    t->arenas_per_wave = e->shape.arenas_per_wave;
+    return RSX_OK;
+}
+
+rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t env_steps, rsx_tune_result* results,
+                        int32_t capacity, int32_t* count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!device_actions || env_steps < 1) return fail(RSX_INVALID_ARG, "need device actions and env_steps >= 1");
+    HIP_TRY(hipSetDevice(e->device));
+    static const int kBlocks[] = {64, 128, 256};
+    static const int kApw[] = {64, 32, 16, 8};
+    std::vector<rsx_tune_result> res;
+    for (int bl : kBlocks)
+        for (int apw : kApw) res.push_back(rsx_tune_result{{bl, apw}, -1.f, 0});
+    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 12");
+    // Snapshot of everything a trial overwrites, restored after each trial
+    // and at the end.
+    std::vector<std::pair<void*, size_t>> bufs;
+    for (auto& b : e->bucket)
+        if (b.n) bufs.push_back({b.d_words, (size_t)b.n * b.words * sizeof(uint32_t)});
+    bufs.push_back({e->d_obs, sizeof(float) * std::max<int64_t>(1, e->obs_floats)});
+    bufs.push_back({e->d_rew, sizeof(float) * std::max(1, e->num_cars)});
+    bufs.push_back({e->d_term, std::max<size_t>(1, (size_t)e->num_arenas)});
+    if (e->cull.enabled) bufs.push_back({e->cull.stats, 3 * sizeof(unsigned long long)});
+    std::vector<void*> saved(bufs.size(), nullptr);
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    rsx_status status = RSX_OK;
+    auto cleanup = [&]() {
+        for (void* p : saved) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    };
+    auto restore = [&]() -> hipError_t {
+        for (size_t i = 0; i < bufs.size(); i++) {
+            hipError_t r = hipMemcpyAsync(bufs[i].first, saved[i], bufs[i].second, hipMemcpyDeviceToDevice, st);
+            if (r != hipSuccess) return r;
+        }
+        return hipSuccess;
+    };
+    auto run = [&]() -> rsx_status {
+        // the trials run on the null stream: a queue of their own would need
+        // its own private-segment (scratch) allocation for these kernels
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        for (size_t i = 0; i < bufs.size(); i++) {
+            HIP_TRY(hipMalloc(&saved[i], bufs[i].second));
+            HIP_TRY(hipMemcpyAsync(saved[i], bufs[i].first, bufs[i].second, hipMemcpyDeviceToDevice, st));
+        }
+        int best = -1;
+        for (size_t k = 0; k < res.size(); k++) {
+            const LaunchShape s{res[k].tuning.block_lanes, res[k].tuning.arenas_per_wave};
+            const size_t lds = arena_lds_per_wave(e->cull.enabled != 0) * (s.block / 64);
+            if (lds > e->lds_per_block_max || prepare_arena_kernels(lds) != hipSuccess) {
+                (void)hipGetLastError();
+                continue;  // does not fit the workgroup's LDS
+            }
+            HIP_TRY(restore());
+            HIP_TRY(env_launch(e, device_actions, s, st));  // warm-up (and code-object load)
+            HIP_TRY(restore());
+            HIP_TRY(hipEventRecord(ev0, st));
+            for (int i = 0; i < env_steps; i++) HIP_TRY(env_launch(e, device_actions, s, st));
+            HIP_TRY(hipEventRecord(ev1, st));
+            HIP_TRY(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+            res[k].ms_per_env_step = ms / (float)env_steps;
+            res[k].feasible = 1;
+            if (best < 0 || res[k].ms_per_env_step < res[best].ms_per_env_step) best = (int)k;
+        }
+        HIP_TRY(restore());
+        HIP_TRY(hipStreamSynchronize(st));
+        if (best < 0) return fail(RSX_HIP_ERROR, "no launch shape fits this device");
+        e->shape = LaunchShape{res[best].tuning.block_lanes, res[best].tuning.arenas_per_wave};
+        e->gen++;
+        return RSX_OK;
+    };
+    status = run();
+    cleanup();
+    if (status != RSX_OK) return status;
+    if (results) std::copy(res.begin(), res.end(), results);
+    if (count) *count = (int32_t)res.size();
+    return RSX_OK;
+}
+
+rsx_status rsx_capture_env_graph(rsx_engine* e, const int32_t* device_actions, int32_t env_steps) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!device_actions || env_steps < 1) return fail(RSX_INVALID_ARG, "need device actions and env_steps >= 1");
+    HIP_TRY(hipSetDevice(e->device));
+    if (e->graph_exec) HIP_TRY(hipGraphExecDestroy(e->graph_exec));
+    if (e->graph) HIP_TRY(hipGraphDestroy(e->graph));
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    RSX_TRY(prepare_shape(e, e->shape));
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t r = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    for (int i = 0; i < env_steps && r == hipSuccess; i++) r = env_launch(e, device_actions, e->shape, cs);
+    hipGraph_t g = nullptr;
+    const hipError_t r_end = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (r == hipSuccess) r = r_end;
+    if (r == hipSuccess) r = hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0);
+    if (r != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        e->graph_exec = nullptr;
+        return fail(RSX_HIP_ERROR, std::string("env-step capture: ") + hipGetErrorString(r));
+    }
+    e->graph = g;
+    e->graph_gen = e->gen;
+    return RSX_OK;
+}
+
+rsx_status rsx_launch_env_graph(rsx_engine* e, void* stream) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!e->graph_exec) return fail(RSX_INVALID_ARG, "no env step captured");
+    if (e->graph_gen != e->gen)
+        return fail(RSX_INVALID_ARG, "the launch shape, env config, action table or cull changed since the capture");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipGraphLaunch(e->graph_exec, (hipStream_t)stream));
     return RSX_OK;
 }
 

@@ -1,0 +1,95 @@
+// rsx_kern.h — launch interface of the arena kernels.
+//
+// Work mapping: one lane owns one arena for a whole launch.  A launch loads
+// the arena's words from the field-major SoA buffer (coalesced across the
+// lanes of a wavefront), runs the requested ticks in registers and private
+// memory, and stores the words back.  The arenas of one team size form a
+// bucket; each bucket has its own instantiation of the tick (MAXC = 2, 4, 6
+// car slots), so no wavefront mixes car counts.
+//
+// Launch shape (chosen by rsx_set_tuning / rsx_autotune):
+//   block            lanes per workgroup: 64, 128 or 256 (1, 2 or 4 wavefronts);
+//   arenas_per_wave  arenas per 64-lane wavefront: 64, 32, 16 or 8.  With
+//                    fewer than 64, lane l runs the arena of lane
+//                    l % arenas_per_wave and stores nothing: the wavefront
+//                    then follows fewer arenas' branches, and more wavefronts
+//                    are in flight to hide memory latency.
+// Each wavefront owns arena_lds_per_wave() bytes of LDS; no code waits for
+// another wavefront of its workgroup.
+//
+// The instantiations live in separate translation units (rsx_k_ticks.hip,
+// rsx_k_env.hip, rsx_k_mixed.hip) so they compile in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/rsx_types.h"
+#include "rsx_env.h"
+#include "rsx_sdf.h"
+
+namespace rsx {
+
+// Arena goal flags word (kArFlags): a goal has been scored this episode, and
+// whether it went into the orange goal's side (ball y <= 0).
+constexpr uint32_t kFlagGoal = 1u, kFlagGoalOrange = 2u;
+
+// One team-size bucket.
+struct BucketArgs {
+    uint32_t* words = nullptr;           // field-major SoA words, n arenas
+    int n = 0;
+    const rsx_controls* ctrl = nullptr;  // n * MAXC controls (tick path)
+    const int* car_off = nullptr;        // global index of each arena's first car
+    const int64_t* obs_off = nullptr;    // float offset of each arena's first obs row
+    const int* global = nullptr;         // global arena index
+};
+
+// The world every bucket shares.
+struct WorldArgs {
+    CarGeom cg;
+    BallGeom bg;
+    MeshView mv;
+    const PadGeom* pads;
+    sdf::Cull cull;
+};
+
+// The env half of a fused env step.
+struct EnvArgs {
+    const int32_t* act;        // device action index per car (global car order)
+    const rsx_controls* lut;   // action table
+    int lut_n;
+    int ticks;                 // tick skip
+    EnvConfig E;
+    float* obs;
+    float* rew;
+    uint8_t* term;
+};
+
+struct LaunchShape {
+    int block = 64;
+    int arenas_per_wave = 64;
+};
+
+// LDS bytes one wavefront needs: the lanes' GJK / convex-cast simplices, or
+// (sharing the space) the cull's activation buffer when the cull is on.
+size_t arena_lds_per_wave(bool cull);
+// Wavefronts of a launch over n arenas.
+int arena_waves(int n, const LaunchShape& s);
+// Whether s is one of the supported shapes.
+bool shape_valid(const LaunchShape& s);
+
+hipError_t launch_ticks(int maxc, const BucketArgs& b, const WorldArgs& w, int ticks, const LaunchShape& s,
+                        hipStream_t st);
+hipError_t launch_env(int maxc, const BucketArgs& b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
+                      hipStream_t st);
+// One launch over every non-empty bucket (b[0] 1v1, b[1] 2v2, b[2] 3v3); the
+// 3v3 wavefronts come first in the grid, so the longest work starts first.
+hipError_t launch_env_mixed(const BucketArgs* b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
+                            hipStream_t st);
+// Sets each arena kernel's dynamic-LDS limit to `bytes` (needed above 64 KiB);
+// call outside stream capture.
+hipError_t prepare_arena_kernels(size_t bytes);
+hipError_t prepare_env_kernels(size_t bytes);
+hipError_t prepare_tick_kernels(size_t bytes);
+hipError_t prepare_mixed_kernel(size_t bytes);
+
+}  // namespace rsx

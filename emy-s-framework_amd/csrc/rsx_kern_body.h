@@ -1,0 +1,162 @@
+// rsx_kern_body.h — device bodies of the arena kernels (one wavefront's
+// share of a launch) and the shape arithmetic shared by their launchers.
+// Included by the kernel translation units only.
+#pragma once
+
+#include <mutex>
+
+#include "rsx_kern.h"
+#include "rsx_soa.h"
+
+namespace rsx {
+
+// The lane's arena within a wavefront whose first arena is `first`:
+// lanes past arenas_per_wave mirror lane (l % arenas_per_wave); lanes past
+// the bucket's end run its last arena.  Neither stores.
+struct LaneArena {
+    int a;
+    bool own;
+};
+
+__device__ inline LaneArena lane_arena(int first, int n, int apw) {
+    const int l = threadIdx.x & 63;
+    const int a0 = first + (l & (apw - 1));
+    return LaneArena{a0 < n ? a0 : n - 1, l < apw && a0 < n};
+}
+
+__device__ inline void tick_ctx(TickCtx& T, const WorldArgs& G, uint8_t* lds_wave, bool own) {
+    T.cg = G.cg;
+    T.bg = G.bg;
+    T.mv = G.mv;
+    T.pads = G.pads;
+    T.dt = 1.f / kTickRate;
+    T.sx = (gjk::Simplex*)lds_wave + (threadIdx.x & 63);
+    T.lds_act = (uint16_t*)lds_wave;
+    T.cull = G.cull;
+    T.own = own;
+}
+
+// `ticks` ticks of the bucket's arenas under their stored controls
+// (RocketSimCudaBatch::Step).  wave: the wavefront's index within the bucket.
+template <int MAXC>
+__device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs& G, int ticks, int wave, int apw,
+                                           uint8_t* lds_wave) {
+    const int first = wave * apw;
+    if (first >= B.n) return;
+    const LaneArena la = lane_arena(first, B.n, apw);
+    const int a = la.a;
+    Col col{B.words + a, B.n};
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, G.cg);
+    for (int i = 0; i < A.ncars; i++) {
+        const rsx_controls& c = B.ctrl[(int64_t)a * MAXC + i];
+        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
+                                     c.handbrake != 0};
+    }
+    TickCtx T;
+    tick_ctx(T, G, lds_wave, la.own);
+    TickScratch<MAXC> W;
+    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
+    uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
+    for (int t = 0; t < ticks; t++) {
+        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
+            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
+    }
+    if (!la.own) return;
+    pack_arena<MAXC>(col, A);
+    col.set(Layout<MAXC>::kArenaBase + kArFlags, flags);
+}
+
+// Fused env step of the bucket's arenas: action indices -> controls, the tick
+// skip, then the stub rewards, the terminal conditions, the kickoff
+// auto-reset of terminal arenas and the DefaultObs rows, all in the lane that
+// owns the arena.
+template <int MAXC>
+__device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G, const EnvArgs& V, int wave, int apw,
+                                         uint8_t* lds_wave) {
+    using L = Layout<MAXC>;
+    const int first = wave * apw;
+    if (first >= B.n) return;
+    const LaneArena la = lane_arena(first, B.n, apw);
+    const int a = la.a;
+    Col col{B.words + a, B.n};
+    Arena<MAXC> A;
+    unpack_arena<MAXC>(col, A, G.cg);
+    float prev[MAXC][kActionFloats];
+    const int c0 = B.car_off[a];
+    for (int i = 0; i < A.ncars; i++) {
+        int k = V.act[c0 + i];
+        k = k < 0 ? 0 : (k >= V.lut_n ? V.lut_n - 1 : k);
+        const rsx_controls c = V.lut[k];
+        A.car[i].controls = Controls{c.throttle, c.steer, c.pitch, c.yaw, c.roll, c.jump != 0, c.boost != 0,
+                                     c.handbrake != 0};
+        prev[i][0] = c.throttle;
+        prev[i][1] = c.steer;
+        prev[i][2] = c.pitch;
+        prev[i][3] = c.yaw;
+        prev[i][4] = c.roll;
+        prev[i][5] = c.jump ? 1.f : 0.f;
+        prev[i][6] = c.boost ? 1.f : 0.f;
+        prev[i][7] = c.handbrake ? 1.f : 0.f;
+    }
+    TickCtx T;
+    tick_ctx(T, G, lds_wave, la.own);
+    TickScratch<MAXC> W;
+    W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
+    uint32_t flags = col.get(L::kArenaBase + kArFlags);
+    for (int t = 0; t < V.ticks; t++) {
+        if (arena_tick(A, T, W) && !(flags & kFlagGoal))
+            flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
+    }
+    if (!la.own) return;
+    const EnvConfig& E = V.E;
+    const bool goal = ball_scored_now(A.ball);
+    bool touched[MAXC];
+    bool any_touch = false;
+    for (int i = 0; i < A.ncars; i++) {
+        touched[i] = touched_in_step(A.car[i], A.tick, V.ticks);
+        any_touch |= touched[i];
+    }
+    float no_touch = col.getf(L::kArenaBase + kArNoTouch);
+    const uint8_t tt = step_terminal(goal, any_touch, (float)V.ticks * (1.f / kTickRate), no_touch, E);
+    for (int i = 0; i < A.ncars; i++) V.rew[c0 + i] = stub_reward<MAXC>(A, i, goal, touched[i], E);
+    uint32_t episode = col.get(L::kArenaBase + kArEpisode);
+    if (tt != kNotTerminal && E.auto_reset) {
+        arena_reset_kickoff<MAXC>(A, G.cg);
+        A.tick = 0;
+        flags = 0;
+        no_touch = 0.f;
+        episode++;
+        for (int i = 0; i < A.ncars; i++)
+            for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
+    }
+    const int row = obs_row_len(A.ncars);
+    float* o = V.obs + B.obs_off[a];
+    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    pack_arena<MAXC>(col, A);
+    col.set(L::kArenaBase + kArFlags, flags);
+    col.setf(L::kArenaBase + kArNoTouch, no_touch);
+    col.set(L::kArenaBase + kArEpisode, episode);
+    V.term[B.global[a]] = tt;
+}
+
+// The workgroup's first wavefront index and this wavefront's LDS.
+__device__ inline int wave_index() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
+
+inline int blocks_for_waves(int waves, const LaunchShape& s) {
+    const int wpb = s.block / 64;
+    return (waves + wpb - 1) / wpb;
+}
+
+// hipFuncSetAttribute once per kernel and size.
+template <typename F>
+hipError_t set_lds_limit(F kernel, size_t bytes, size_t& done) {
+    static std::mutex m;
+    std::lock_guard<std::mutex> g(m);
+    if (bytes <= done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess) done = bytes;
+    return e;
+}
+
+}  // namespace rsx

@@ -207,4 +207,25 @@ RSX_HD void susp_cell_indices(V3 pos_uu, int& i, int& j, int& k) {
 }
 RSX_HD int susp_cell(int i, int j, int k) { return (i * kSuspCellsY * kSuspCellsZ) + (j * kSuspCellsZ) + k; }
 
+// Broadphase cell of a point (btRSBroadphase::GetCellIdx, clamped to the grid).
+RSX_HD int bp_cell_index(const MeshView& w, V3 pos) {
+    V3 f = (pos - V3{kBpMinX, kBpMinY, kBpMinZ}) / kBpCellSize;
+    int i = (int)f.x, j = (int)f.y, k = (int)f.z;
+    i = i < 0 ? 0 : (i > w.bp_nx - 1 ? w.bp_nx - 1 : i);
+    j = j < 0 ? 0 : (j > w.bp_ny - 1 ? w.bp_ny - 1 : j);
+    k = k < 0 ? 0 : (k > w.bp_nz - 1 ? w.bp_nz - 1 : k);
+    return i * w.bp_ny * w.bp_nz + j * w.bp_nz + k;
+}
+
+// Whether a body registered at cell `reg` is listed in cell `cell`
+// (btRSBroadphase _UpdateCellsDynamic: the 3x3x3 block around reg).
+RSX_HD bool bp_listed(const MeshView& w, int reg, int cell) {
+    if (reg < 0) return false;
+    const int yz = w.bp_ny * w.bp_nz;
+    const int ri = reg / yz, rj = (reg / w.bp_nz) % w.bp_ny, rk = reg % w.bp_nz;
+    const int ci = cell / yz, cj = (cell / w.bp_nz) % w.bp_ny, ck = cell % w.bp_nz;
+    const int di = ri - ci, dj = rj - cj, dk = rk - ck;
+    return di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1;
+}
+
 }  // namespace rsx

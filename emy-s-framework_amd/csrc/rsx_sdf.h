@@ -32,7 +32,7 @@ constexpr int kHidden = 128;
 constexpr int kTiles = kHidden / 16;                  // 16-wide tiles per layer side
 constexpr int kSwz = kTiles * kTiles * 64 * 4;        // bf16 values of one swizzled 128x128 operand
 constexpr int kActStride = kHidden + 8;               // LDS row stride (bf16), padded against bank conflicts
-constexpr int kLdsActBytes = 2 * 64 * kActStride * 2; // two activation buffers of 64 rows
+constexpr int kLdsActBytes = 64 * kActStride * 2;     // one activation buffer of 64 rows (layers run in place)
 constexpr float kOutScale = 100.f;                    // network output unit (uu)
 
 // FP32 parameter vector layout (the master copy the optimiser updates).
@@ -160,9 +160,20 @@ RSX_HD float point_tri_dist2(V3 p, V3 a, V3 b, V3 c) {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Orders the LDS accesses of one wavefront: writes issued before it are seen
+// by every lane's reads after it.  Synchronises nothing beyond the wavefront,
+// so wavefronts of one workgroup may take different branches around it.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One 128 -> 128 layer for the 64 rows in `in` (LDS, bf16, row stride
 // kActStride): out = SiLU(in x B + bias) in bf16 (and, if z_out is given,
-// the pre-activation for row r, column n at z_out[r * 128 + n]).
+// the pre-activation for row r, column n at z_out[r * 128 + n]).  out may be
+// in: each 16-row tile is read whole (into the matrix operands) before any
+// of its rows is written, and no tile reads another tile's rows.
 // Every lane of the wavefront must call it.
 __device__ inline void mfma_layer(const uint16_t* in, uint16_t* out, const uint16_t* bswz, const float* bias,
                                   float* z_out) {
@@ -179,6 +190,7 @@ __device__ inline void mfma_layer(const uint16_t* in, uint16_t* out, const uint1
                 acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
             }
         }
+        wave_sync();
         for (int nt = 0; nt < kTiles; nt++) {
             const int n = nt * 16 + r16;
             const float bn = bias[n];
@@ -193,26 +205,26 @@ __device__ inline void mfma_layer(const uint16_t* in, uint16_t* out, const uint1
 }
 
 // Network output (distance, uu) for this lane's point p (uu).  The
-// wavefront's 64 lanes each bring one point; lds holds kLdsActBytes bytes.
-// Every lane of the wavefront must call it.
+// wavefront's 64 lanes each bring one point; lds holds kLdsActBytes bytes of
+// this wavefront's own.  Every lane of the wavefront must call it; other
+// wavefronts of the workgroup are not waited for.
 __device__ inline float eval_wave(const Net& net, V3 p, uint16_t* lds) {
     const int l = threadIdx.x & 63;
-    uint16_t* buf0 = lds;
-    uint16_t* buf1 = lds + 64 * kActStride;
     const float x = (p.x - net.in.cx) / net.in.hx, y = (p.y - net.in.cy) / net.in.hy,
                 z = (p.z - net.in.cz) / net.in.hz;
+    wave_sync();
     for (int o = 0; o < kHidden; o++) {
         float zz = net.w1[o * 3 + 0] * x + net.w1[o * 3 + 1] * y + net.w1[o * 3 + 2] * z + net.b1[o];
-        buf0[l * kActStride + o] = f32_to_bf16(silu(zz));
+        lds[l * kActStride + o] = f32_to_bf16(silu(zz));
     }
-    __syncthreads();
-    mfma_layer(buf0, buf1, net.w2s, net.b2, nullptr);
-    __syncthreads();
-    mfma_layer(buf1, buf0, net.w3s, net.b3, nullptr);
-    __syncthreads();
+    wave_sync();
+    mfma_layer(lds, lds, net.w2s, net.b2, nullptr);
+    wave_sync();
+    mfma_layer(lds, lds, net.w3s, net.b3, nullptr);
+    wave_sync();
     float s = *net.b4;
-    for (int k = 0; k < kHidden; k++) s += net.w4[k] * bf16_to_f32(buf0[l * kActStride + k]);
-    __syncthreads();
+    for (int k = 0; k < kHidden; k++) s += net.w4[k] * bf16_to_f32(lds[l * kActStride + k]);
+    wave_sync();
     return s * kOutScale;
 }
 #endif

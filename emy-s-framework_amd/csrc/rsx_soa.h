@@ -13,9 +13,13 @@
 //
 // Car words (Bullet units for pose/velocities, seconds for timers):
 //   0-2 pos, 3-11 basis rows, 12-14 vel, 15-17 ang vel, 18-29 timers/scalars,
-//   30-32 flip torque, 33-35 world contact normal, 36 flag bits, 37 id,
+//   30-32 flip torque, 33-35 world contact normal, 36 flag bits (bit 19: the
+//   body responds to contacts as of its last pre-tick), 37 id,
 //   38 contact other id, 39-42 ball-hit ticks, 43-51 ball-hit vectors,
-//   52-56 last controls (analog), 57-80 per-wheel carried values (4 x 6).
+//   52-56 last controls (analog), 57-80 per-wheel carried values (4 x 6),
+//   81 broadphase registration cell.
+// Arena words: tick (2), RNG, car count, no-touch time, episode, goal flags,
+//   ball broadphase registration cell.
 #pragma once
 
 #include "rsx_step.h"
@@ -36,7 +40,8 @@ struct Layout {
 };
 
 // Arena record words
-constexpr int kArTickLo = 0, kArTickHi = 1, kArRng = 2, kArNcars = 3, kArNoTouch = 4, kArEpisode = 5, kArFlags = 6;
+constexpr int kArTickLo = 0, kArTickHi = 1, kArRng = 2, kArNcars = 3, kArNoTouch = 4, kArEpisode = 5, kArFlags = 6,
+              kArBallReg = 7;
 
 RSX_HD uint32_t f2u(float f) {
     union {
@@ -114,6 +119,7 @@ RSX_HD void unpack_car(const Col& c, int b, Car& k, const CarGeom& g) {
     k.last_controls.jump = (f >> 16) & 1;
     k.last_controls.boost = (f >> 17) & 1;
     k.last_controls.handbrake = (f >> 18) & 1;
+    k.active = (f >> 19) & 1;
     k.id = c.get(b + 37);
     k.car_contact_other_id = c.get(b + 38);
     k.ball_hit_tick = c.get64(b + 39);
@@ -136,6 +142,7 @@ RSX_HD void unpack_car(const Col& c, int b, Car& k, const CarGeom& g) {
         wh.long_friction = c.getf(o + 4);
         wh.extra_pushback = c.getf(o + 5);
     }
+    k.bp_reg = (int)c.get(b + 81);
     // derived
     car_update_inertia(k, g);
     k.force = k.torque = k.vel_cache = V3{0, 0, 0};
@@ -180,6 +187,7 @@ RSX_HD void pack_car(const Col& c, int b, const Car& k) {
     f |= (uint32_t)k.last_controls.jump << 16;
     f |= (uint32_t)k.last_controls.boost << 17;
     f |= (uint32_t)k.last_controls.handbrake << 18;
+    f |= (uint32_t)k.active << 19;
     c.set(b + 36, f);
     c.set(b + 37, k.id);
     c.set(b + 38, k.car_contact_other_id);
@@ -203,6 +211,7 @@ RSX_HD void pack_car(const Col& c, int b, const Car& k) {
         c.setf(o + 4, wh.long_friction);
         c.setf(o + 5, wh.extra_pushback);
     }
+    c.set(b + 81, (uint32_t)k.bp_reg);
 }
 
 template <int MAXC>
@@ -219,6 +228,7 @@ RSX_HD void unpack_arena(const Col& c, Arena<MAXC>& A, const CarGeom& g) {
     A.ball.ang = c.getv(L::kBallBase + 6);
     A.ball.force = A.ball.vel_cache = V3{0, 0, 0};
     A.ball.interp = A.ball.xf;
+    A.ball.bp_reg = (int)c.get(L::kArenaBase + kArBallReg);
     for (int p = 0; p < kNumPads; p++) {
         int o = L::kPadBase + p * kPadWords;
         A.pad[p].is_active = c.get(o) != 0;
@@ -238,6 +248,7 @@ RSX_HD void pack_arena(const Col& c, const Arena<MAXC>& A) {
     c.setv(L::kBallBase + 0, A.ball.xf.o);
     c.setv(L::kBallBase + 3, A.ball.vel);
     c.setv(L::kBallBase + 6, A.ball.ang);
+    c.set(L::kArenaBase + kArBallReg, (uint32_t)A.ball.bp_reg);
     for (int p = 0; p < kNumPads; p++) {
         int o = L::kPadBase + p * kPadWords;
         c.set(o, A.pad[p].is_active ? 1u : 0u);

@@ -312,6 +312,9 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             car_shape_aabb(g, A.car[i].interp, b0, b1);
             broad_aabb(a0, a1, b0, b1, W.bmin[1 + i], W.bmax[1 + i]);
         }
+        // btRSBroadphase::setAabb: the registrations follow the new AABBs
+        A.ball.bp_reg = bp_cell_index(T.mv, W.bmin[0]);
+        for (int i = 0; i < A.ncars; i++) A.car[i].bp_reg = bp_cell_index(T.mv, W.bmin[1 + i]);
     }
 
     // ---- pairs, manifolds, contacts (dispatch order) ----
@@ -727,7 +730,7 @@ __device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAX
             if (any) {
                 const float d = sdf::eval_wave(C.net, q * kBT2UU, T.lds_act);
                 if (need) W.cull_pred[b] = d;
-                nq += __popcll(any);
+                nq += __popcll(__ballot(need && T.own));
             }
         }
         const float reach = b == 0 ? C.reach_ball_uu : C.reach_car_uu;
@@ -735,8 +738,8 @@ __device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAX
         const bool cull = have && (W.cull_pred[b] - C.margin_uu > reach + step * (float)W.cull_age);
         far[b] = cull;
         const bool defect = cull && C.audit && mesh_within(T.mv, q, reach * kUU2BT);
-        nc += __popcll(__ballot(cull));
-        nd += __popcll(__ballot(defect));
+        nc += __popcll(__ballot(cull && T.own));
+        nd += __popcll(__ballot(defect && T.own));
     }
     W.cull_age++;
     if ((threadIdx.x & 63) == 0) {

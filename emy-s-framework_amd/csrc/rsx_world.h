@@ -106,9 +106,17 @@ struct Car {
     uint64_t ball_hit_tick, ball_hit_extra_tick;
     V3 ball_hit_rel_pos, ball_hit_ball_pos, ball_hit_extra_vel;
     Controls last_controls;
-    // tick bookkeeping
-    bool active;               // not demoed this tick (takes part in the world step)
-    int touched_tick;          // reserved
+    // The body responds to contacts (Bullet: no CF_NO_CONTACT_RESPONSE): set by
+    // the car's own pre-tick (false while demolished, Car.cpp:68-82) and kept
+    // until its next one, so the wheel rays of cars stepped before it in a
+    // tick see last tick's value.  Stored with the car; true for a new car.
+    bool active;
+    // Broadphase registration (btRSBroadphase): a dynamic body is listed in the
+    // 27 cells around the cell of its AABB minimum as of its last AABB update
+    // (the previous world step, or when it was added), and a suspension ray
+    // only meets bodies listed in the cell it starts in.  Setting a state
+    // does not move the registration.  Packed cell index; -1: listed nowhere.
+    int bp_reg;
 };
 
 struct Ball {
@@ -118,6 +126,7 @@ struct Ball {
     V3 force;
     V3 vel_cache;
     bool sleeping;
+    int bp_reg;                // broadphase registration, as for Car::bp_reg
 };
 
 struct Pad {

@@ -87,3 +87,13 @@ CARS_ALTERNATING = 1
 assert C.sizeof(CarControls) == 24
 assert C.sizeof(CarState) == 248
 assert C.sizeof(ArenaInfo) == 32
+
+
+class Tuning(C.Structure):
+    """rsx_tuning (include/rsx.h): launch shape of the arena kernels."""
+    _fields_ = [("block_lanes", C.c_int32), ("arenas_per_wave", C.c_int32)]
+
+
+class TuneResult(C.Structure):
+    """rsx_tune_result (include/rsx.h)."""
+    _fields_ = [("tuning", Tuning), ("ms_per_env_step", C.c_float), ("feasible", C.c_int32)]

@@ -30,6 +30,14 @@ def _view(arr):
         return np.ctypeslib.as_array(arr)
 
 
+def mixed_team_sizes(n, seed):
+    """Team size per arena for the mixed workload: 1v1 for half of the arenas,
+    2v2 for 30%, 3v3 for the rest, shuffled by numpy PCG64 seeded with `seed`."""
+    t = np.array([1] * (n // 2) + [2] * (3 * n // 10) + [3] * (n - n // 2 - 3 * n // 10))
+    np.random.default_rng(seed).shuffle(t)
+    return [int(x) for x in t]
+
+
 def random_field_states(batch, seed):
     """Applies the random-state workload to every arena of `batch`."""
     rng = np.random.default_rng(seed)

@@ -175,6 +175,46 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream);
  * (RocketSimCuda.h:345). */
 rsx_status rsx_synchronize(rsx_engine* e);
 
+/* Launch shape of the arena kernels (rsx_step_env, rsx_step_ticks).  The
+ * reference fixes it at 128 threads per block, one thread per arena
+ * (RocketSimCudaApi.cuh:553-554).  Here:
+ *   block_lanes      lanes per workgroup: 64, 128 or 256;
+ *   arenas_per_wave  arenas per 64-lane wavefront: 64, 32, 16 or 8 (below 64
+ *                    the spare lanes repeat a lane's arena and store nothing,
+ *                    which trades lanes for fewer diverging branches per
+ *                    wavefront and more wavefronts in flight).
+ * Results do not depend on the shape.  Default {64, 64}. */
+typedef struct rsx_tuning {
+    int32_t block_lanes;
+    int32_t arenas_per_wave;
+} rsx_tuning;
+
+typedef struct rsx_tune_result {
+    rsx_tuning tuning;
+    float ms_per_env_step;  /* mean over the timed env steps; < 0 when infeasible */
+    int32_t feasible;       /* 0: the shape's LDS does not fit a workgroup */
+} rsx_tune_result;
+
+rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t);
+rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t);
+
+/* Autotuner: times `env_steps` env steps (after one untimed) of every shape
+ * in {64, 128, 256} x {64, 32, 16, 8} on the engine's current arenas with the
+ * given device actions, keeps the fastest, and restores the arenas, the env
+ * outputs and the cull's counters to what they were.  results (nullable,
+ * capacity >= 12) receives every shape's time; *count the number written.
+ * Blocks the host until done. */
+rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t env_steps, rsx_tune_result* results,
+                        int32_t capacity, int32_t* count);
+
+/* hipGraph of the env step: rsx_capture_env_graph records `env_steps`
+ * consecutive rsx_step_env calls reading `device_actions` (the caller
+ * refreshes that buffer between launches); rsx_launch_env_graph replays them
+ * on `stream`.  A change of launch shape, env config, action table or cull
+ * after the capture makes the launch fail until the next capture. */
+rsx_status rsx_capture_env_graph(rsx_engine* e, const int32_t* device_actions, int32_t env_steps);
+rsx_status rsx_launch_env_graph(rsx_engine* e, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,3 +29,11 @@ def cpucheck():
     assert w, "mesh load failed"
     yield lib, w
     lib.cc_world_free(w)
+
+
+@pytest.fixture(scope="session")
+def torch_cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch

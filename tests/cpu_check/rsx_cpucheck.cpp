@@ -53,10 +53,13 @@ void* cc_arena_new(void* world, int n_blue, int n_orange) {
         Car& c = a->A.car[i];
         c.team = i < n_blue ? 0 : 1;
         c.id = (uint32_t)(i + 1);
+        c.active = true;
+        c.bp_reg = -1;  // registered where a state set before the first tick cannot see it
         car_set_spawn<6>(c, a->world->w.cg, 0, 0, kCarSpawnRestZ, 0, kBoostSpawnAmount);
     }
     a->A.ball.xf = xf_identity();
     a->A.ball.xf.o = V3{0, 0, kBallRestZ * kUU2BT};
+    a->A.ball.bp_reg = -1;
     for (int p = 0; p < kNumPads; p++) a->A.pad[p] = Pad{true, 0.f, 0, 0};
     return a;
 }
@@ -138,6 +141,7 @@ int cc_step(void* p, int ticks) {
     T.sx = &sx;
     T.cull = sdf::Cull{};
     T.lds_act = nullptr;
+    T.own = true;
     int scored = 0;
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;
@@ -167,6 +171,8 @@ void cc_soa_roundtrip(void* p, int fill) {
     };
     scribble(&a->A, sizeof(a->A));
     scribble(&a->W, sizeof(a->W));
+    // flags the load must restore, set to the value that breaks a missing load
+    for (int i = 0; i < 6; i++) a->A.car[i].active = (fill >> i) & 1;
     unpack_arena<6>(c, a->A, a->world->w.cg);
     for (int i = 0; i < a->A.ncars; i++) a->A.car[i].controls = keep[i];
 }

@@ -117,7 +117,10 @@ def run_oracle(lib, case):
         lib.ref_arena_free(h)
 
 
-def run_cpucheck(lib, world, case):
+def run_cpucheck(lib, world, case, boundary_every=0, fill=0):
+    """The CPU build of the tick over the case; with boundary_every > 0 the
+    arena goes through the SoA layout (pack, scribbled working state, load)
+    every that many ticks, as at the start of each GPU launch."""
     nb, no = case["blue"], case["orange"]
     h = C.c_void_p(lib.cc_arena_new(world, nb, no))
     try:
@@ -137,6 +140,8 @@ def run_cpucheck(lib, world, case):
         for t in range(case["ticks"]):
             for i, c in enumerate(_ctrls(case, t)):
                 lib.cc_set_controls(h, i, C.byref(c))
+            if boundary_every and t % boundary_every == 0:
+                lib.cc_soa_roundtrip(h, fill + t)
             sc = lib.cc_step(h, 1)
             cars = []
             for i in range(nb + no):

@@ -121,6 +121,7 @@ def load_cpucheck():
         ("cc_set_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
         ("cc_get_wheels", None, [VP, C.c_int, C.POINTER(C.c_float)]),
         ("cc_step", C.c_int, [VP, C.c_int]),
+        ("cc_soa_roundtrip", None, [VP, C.c_int]),
         ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
         ("cc_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
         ("cc_tri_info", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),

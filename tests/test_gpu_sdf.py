@@ -17,14 +17,6 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def torch_cuda():
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no HIP device")
-    return torch
-
-
-@pytest.fixture(scope="module")
 def net(torch_cuda):
     eng = R.import_engine()
     from emy_s_framework_amd.sdf import NeuralSdf
