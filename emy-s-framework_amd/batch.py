@@ -56,6 +56,9 @@ _SIGS = [
     ("rsx_copy_terminals", _I, [_VP, C.POINTER(C.c_uint8), _I]),
     ("rsx_step_ticks", _I, [_VP, _I, _VP]),
     ("rsx_synchronize", _I, [_VP]),
+    ("rsx_set_action_masks", _I, [_VP, _I]),
+    ("rsx_action_masks_buffer", _I, [_VP, C.POINTER(_VP)]),
+    ("rsx_copy_action_masks", _I, [_VP, C.POINTER(C.c_uint8), C.c_int64]),
     ("rsx_set_tuning", _I, [_VP, C.POINTER(Tuning)]),
     ("rsx_get_tuning", _I, [_VP, C.POINTER(Tuning)]),
     ("rsx_autotune", _I, [_VP, _VP, _I, C.POINTER(TuneResult), _I, C.POINTER(_I)]),
@@ -330,3 +333,18 @@ class RocketSimBatch:
 
     def LaunchEnvGraph(self, stream=None):
         _check(self._lib.rsx_launch_env_graph(self._h, stream))
+
+    def SetActionMasks(self, enabled=True):
+        """DefaultAction masks written by StepEnv / BuildObs (90 bytes per car)."""
+        _check(self._lib.rsx_set_action_masks(self._h, int(enabled)))
+
+    def ActionMasksBuffer(self):
+        p = _VP()
+        _check(self._lib.rsx_action_masks_buffer(self._h, C.byref(p)))
+        return p.value
+
+    def CopyBuiltDefaultActionMasks(self):
+        import numpy as np
+        out = np.zeros((self.num_cars, 90), dtype=np.uint8)
+        _check(self._lib.rsx_copy_action_masks(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size))
+        return out

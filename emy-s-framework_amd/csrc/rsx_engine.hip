@@ -93,6 +93,10 @@ struct rsx_engine {
     LaunchShape shape;                 // launch shape of the arena kernels
     uint64_t gen = 1;                  // bumped by every change that alters the env step's launches
     size_t lds_per_block_max = 65536;  // device limit of a workgroup's LDS
+    // DefaultAction masks: tables (ground, air, boost, jump) and one row per car
+    uint8_t* d_mask_tables = nullptr;
+    uint8_t* d_masks = nullptr;
+    bool masks_on = false;
     // captured env steps (rsx_capture_env_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -140,7 +144,8 @@ __global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint
 // observation after a reset).
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_env_obs(uint32_t* words, int n, CarGeom cg, EnvConfig E, float* obs,
-                                                const int64_t* obs_off) {
+                                                const int64_t* obs_off, const int* car_off, uint8_t* masks,
+                                                const uint8_t* mask_tables) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     Col col{words + a, n};
@@ -152,6 +157,9 @@ __global__ void __launch_bounds__(64) k_env_obs(uint32_t* words, int n, CarGeom 
     const int row = obs_row_len(A.ncars);
     float* o = obs + obs_off[a];
     for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    if (masks)
+        for (int i = 0; i < A.ncars; i++)
+            action_mask_row(A.car[i], mask_tables, masks + (int64_t)(car_off[a] + i) * kNumDefaultActions);
 }
 
 // Discrete action expansion: one lane per car slot of the bucket.
@@ -250,6 +258,8 @@ hipError_t env_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, h
     v.obs = e->d_obs;
     v.rew = e->d_rew;
     v.term = e->d_term;
+    v.masks = e->masks_on ? e->d_masks : nullptr;
+    v.mask_tables = e->d_mask_tables;
     int nonempty = 0;
     for (const Bucket& b : e->bucket) nonempty += b.n > 0;
     if (nonempty > 1) {
@@ -402,6 +412,7 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         HIP_TRY(hipMemset(e->d_obs, 0, sizeof(float) * std::max<int64_t>(1, off)));
         HIP_TRY(hipMemset(e->d_rew, 0, sizeof(float) * (size_t)e->num_cars));
         HIP_TRY(hipMemset(e->d_term, 0, (size_t)e->num_arenas));
+        if ((s = upload(&e->d_mask_tables, default_action_masks())) != RSX_OK) return s;
     }
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
@@ -456,6 +467,8 @@ rsx_status rsx_destroy(rsx_engine* e) {
     (void)hipFree(e->d_rew);
     (void)hipFree(e->d_term);
     (void)hipFree(e->d_lut);
+    (void)hipFree(e->d_mask_tables);
+    (void)hipFree(e->d_masks);
     (void)hipFree(e->d_idx_stage);
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
@@ -900,12 +913,14 @@ rsx_status rsx_build_obs(rsx_engine* e, void* stream) {
         Bucket& b = e->bucket[k];
         if (!b.n) continue;
         dim3 grid((b.n + 63) / 64), block(64);
-        if (b.maxc == 2)
-            hipLaunchKernelGGL(k_env_obs<2>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
-        else if (b.maxc == 4)
-            hipLaunchKernelGGL(k_env_obs<4>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
-        else
-            hipLaunchKernelGGL(k_env_obs<6>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off);
+        uint8_t* masks = e->masks_on ? e->d_masks : nullptr;
+#define RSX_LAUNCH_OBS(M)                                                                                        \
+    hipLaunchKernelGGL(k_env_obs<M>, grid, block, 0, st, b.d_words, b.n, e->hw.cg, e->env, e->d_obs, b.d_obs_off, \
+                       b.d_car_off, masks, e->d_mask_tables)
+        if (b.maxc == 2) RSX_LAUNCH_OBS(2);
+        else if (b.maxc == 4) RSX_LAUNCH_OBS(4);
+        else RSX_LAUNCH_OBS(6);
+#undef RSX_LAUNCH_OBS
         HIP_TRY(hipGetLastError());
     }
     return RSX_OK;
@@ -945,6 +960,39 @@ rsx_status rsx_copy_rewards(rsx_engine* e, float* out, int32_t count) {
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(out, e->d_rew, sizeof(float) * count, hipMemcpyDeviceToHost));
+    return RSX_OK;
+}
+
+rsx_status rsx_set_action_masks(rsx_engine* e, int32_t enabled) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (enabled && e->lut_n != kNumDefaultActions)
+        return fail(RSX_INVALID_ARG, "action masks follow the default 90-entry action table");
+    HIP_TRY(hipSetDevice(e->device));
+    if (enabled && !e->d_masks) {
+        const size_t bytes = (size_t)kNumDefaultActions * std::max(1, e->num_cars);
+        HIP_TRY(hipMalloc((void**)&e->d_masks, bytes));
+        HIP_TRY(hipMemset(e->d_masks, 0, bytes));
+    }
+    e->masks_on = enabled != 0;
+    e->gen++;
+    return RSX_OK;
+}
+
+rsx_status rsx_action_masks_buffer(rsx_engine* e, uint8_t** d_masks) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!d_masks) return fail(RSX_INVALID_ARG, "null output");
+    *d_masks = e->d_masks;
+    return RSX_OK;
+}
+
+rsx_status rsx_copy_action_masks(rsx_engine* e, uint8_t* out, int64_t count) {
+    if (!e || !out) return fail(RSX_INVALID_ARG, "null argument");
+    if (!e->d_masks) return fail(RSX_INVALID_ARG, "action masks are off");
+    if (count != (int64_t)kNumDefaultActions * e->num_cars)
+        return fail(RSX_INVALID_ARG, "count must equal 90 x the number of cars");
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, e->d_masks, (size_t)count, hipMemcpyDeviceToHost));
     return RSX_OK;
 }
 

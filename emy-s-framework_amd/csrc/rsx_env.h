@@ -48,6 +48,25 @@ RSX_HD bool has_flip_or_jump(const Car& c) {
     return c.on_ground || (!c.has_flipped && !c.has_double_jumped && c.air_time_since_jump < kDoubleJumpMaxDelay);
 }
 
+constexpr int kNumDefaultActions = 90;
+
+// DefaultAction::GetActionMask (DefaultAction.cpp:91-118) for one car:
+// tables holds the ground, air, boost and jump masks of the default table.
+RSX_HD void action_mask_row(const Car& c, const uint8_t* tables, uint8_t* out) {
+    const uint8_t* base = c.on_ground ? tables : tables + kNumDefaultActions;
+    const uint8_t* boost = tables + 2 * kNumDefaultActions;
+    const uint8_t* jump = tables + 3 * kNumDefaultActions;
+    const bool no_boost = c.boost == 0;
+    const bool turtled = c.world_contact && c.world_contact_normal.z > 0.9f;
+    const bool can_jump = has_flip_or_jump(c) || turtled;
+    for (int i = 0; i < kNumDefaultActions; i++) {
+        uint8_t m = base[i];
+        if (no_boost && boost[i]) m = 0;
+        if (can_jump && jump[i]) m = 1;
+        out[i] = m;
+    }
+}
+
 RSX_HD bool ball_scored_now(const Ball& b) {
     return fabsf(b.xf.o.y * kBT2UU) > (kGoalThresholdY + kBallRadius);
 }

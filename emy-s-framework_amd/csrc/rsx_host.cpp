@@ -603,7 +603,7 @@ bool obs_pad_map(const PadGeom* pads, int8_t* out) {
 // follows steer), then 66 air actions (pitch x yaw x roll x jump x boost, no
 // yaw with jump, no duplicate of the all-zero ground rows; throttle follows
 // boost, steer follows yaw, handbrake on for jumps with a direction).
-std::vector<rsx_controls> default_action_lut() {
+std::vector<rsx_controls> default_action_lut(int* ground_actions) {
     std::vector<rsx_controls> out;
     const float tri[3] = {-1, 0, 1};
     auto push = [&](float thr, float steer, float pitch, float yaw, float roll, int jump, int boost, int hb) {
@@ -625,6 +625,7 @@ std::vector<rsx_controls> default_action_lut() {
                     if (boost && thr != 1) continue;
                     push(thr, steer, 0, steer, 0, 0, boost, hb);
                 }
+    if (ground_actions) *ground_actions = (int)out.size();
     for (float pitch : tri)
         for (float yaw : tri)
             for (float roll : tri)
@@ -636,6 +637,28 @@ std::vector<rsx_controls> default_action_lut() {
                         push((float)boost, yaw, pitch, yaw, roll, jump, boost, hb);
                     }
     return out;
+}
+
+// DefaultAction's constructor (DefaultAction.cpp:60-89): which entries a
+// grounded car, an airborne car, boost and jump make available.  The air
+// mask starts after the first aerial entry (strict comparison in the
+// reference) and takes in the ground entries whose throttle equals their
+// boost and whose handbrake equals (yaw != 0).
+std::vector<uint8_t> default_action_masks() {
+    int ng = 0;
+    const std::vector<rsx_controls> lut = default_action_lut(&ng);
+    const int n = (int)lut.size();
+    std::vector<uint8_t> m(4 * (size_t)n, 0);
+    uint8_t *ground = m.data(), *air = ground + n, *boost = air + n, *jump = boost + n;
+    for (int i = 0; i < n; i++) {
+        const rsx_controls& a = lut[i];
+        jump[i] = a.jump != 0;
+        boost[i] = a.boost != 0;
+        ground[i] = i < ng;
+        air[i] = i > ng && !a.jump;
+        if (i < ng && a.throttle == (float)a.boost && (a.yaw != 0.f) == (a.handbrake != 0)) air[i] = 1;
+    }
+    return m;
 }
 
 // Car::SetState (Car.cpp:23-36): transform, velocities (uu -> BT), inertia.

@@ -51,6 +51,9 @@ rsx_controls controls_to_public(const Controls& c);
 bool obs_pad_map(const PadGeom* pads, int8_t* out34);
 
 // RLGym DefaultAction's 90-entry discrete action table, in its order.
-std::vector<rsx_controls> default_action_lut();
+std::vector<rsx_controls> default_action_lut(int* ground_actions = nullptr);
+// DefaultAction's masks over the default table, kNumDefaultActions bytes each,
+// in the order ground, air, boost, jump.
+std::vector<uint8_t> default_action_masks();
 
 }  // namespace rsx

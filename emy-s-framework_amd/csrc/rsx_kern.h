@@ -62,6 +62,8 @@ struct EnvArgs {
     float* obs;
     float* rew;
     uint8_t* term;
+    uint8_t* masks;               // DefaultAction mask rows (kNumDefaultActions bytes per car), or null
+    const uint8_t* mask_tables;   // ground, air, boost, jump masks of the default table
 };
 
 struct LaunchShape {

@@ -69,8 +69,8 @@ __device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs&
 
 // Fused env step of the bucket's arenas: action indices -> controls, the tick
 // skip, then the stub rewards, the terminal conditions, the kickoff
-// auto-reset of terminal arenas and the DefaultObs rows, all in the lane that
-// owns the arena.
+// auto-reset of terminal arenas, the DefaultObs rows and (when requested)
+// the DefaultAction mask rows, all in the lane that owns the arena.
 template <int MAXC>
 __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G, const EnvArgs& V, int wave, int apw,
                                          uint8_t* lds_wave) {
@@ -133,6 +133,9 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     const int row = obs_row_len(A.ncars);
     float* o = V.obs + B.obs_off[a];
     for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    if (V.masks)
+        for (int i = 0; i < A.ncars; i++)
+            action_mask_row(A.car[i], V.mask_tables, V.masks + (int64_t)(c0 + i) * kNumDefaultActions);
     pack_arena<MAXC>(col, A);
     col.set(L::kArenaBase + kArFlags, flags);
     col.setf(L::kArenaBase + kArNoTouch, no_touch);

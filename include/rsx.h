@@ -166,6 +166,19 @@ rsx_status rsx_copy_obs(rsx_engine* e, float* out, int64_t count);
 rsx_status rsx_copy_rewards(rsx_engine* e, float* out, int32_t count);
 rsx_status rsx_copy_terminals(rsx_engine* e, uint8_t* out, int32_t count);
 
+/* DefaultAction masks (RLGym DefaultAction::GetActionMask,
+ * DefaultAction.cpp:91-118), the masks half of BuildAdvancedObsAndDefaultMasks
+ * (RocketSimCuda.h:316).  Off by default.  When on, rsx_step_env and
+ * rsx_build_obs also write, per car, 90 bytes (1 = the action of that index
+ * in the default table is allowed) from the state the obs rows describe.
+ * Needs the default 90-entry action table.  rsx_action_masks_buffer gives the
+ * device buffer (num_cars x 90, car-major, stable once allocated; replaces
+ * GetBuiltDefaultActionMasksDevicePtr, RocketSimCuda.h:394);
+ * rsx_copy_action_masks replaces CopyBuiltDefaultActionMasks (342). */
+rsx_status rsx_set_action_masks(rsx_engine* e, int32_t enabled);
+rsx_status rsx_action_masks_buffer(rsx_engine* e, uint8_t** d_masks);
+rsx_status rsx_copy_action_masks(rsx_engine* e, uint8_t* out, int64_t count);
+
 /* Step every arena by `ticks` physics ticks; replaces Step
  * (RocketSimCuda.h:310, RocketSimCudaApi.cuh:551-571).  Asynchronous on
  * `stream` (a hipStream_t, NULL = default stream). */

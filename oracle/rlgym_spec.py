@@ -21,6 +21,7 @@ Reference lines followed:
                                           RewardCore/CommonRewards.h:33-99
   GoalScoreCondition / NoTouchCondition   TerminalConditions/GoalScoreCondition.h, NoTouchCondition.h
   reward sum and terminal priority        EnvSet/EnvSet.cpp:169-242
+  DefaultAction masks                     ActionParsers/DefaultAction.cpp:60-118
 """
 import numpy as np
 
@@ -149,3 +150,32 @@ def step_terminal(goal, any_touch, step_dt, no_touch, timeout, goal_terminal=Tru
             if no_touch >= timeout and t == 0:
                 t = 2
     return t, no_touch
+
+
+def default_action_masks(lut, ground_actions=24):
+    """The four masks DefaultAction's constructor builds over the table `lut`
+    (rows of throttle, steer, pitch, yaw, roll, jump, boost, handbrake):
+    (ground, air, boost, jump), each a list of 0/1 per entry."""
+    n = len(lut)
+    ground, air, boost, jump = [0] * n, [0] * n, [0] * n, [0] * n
+    for i, (thr, steer, pitch, yaw, roll, j, b, hb) in enumerate(lut):
+        jump[i] = int(bool(j))
+        boost[i] = int(bool(b))
+        ground[i] = int(i < ground_actions)
+        if i > ground_actions and not j:
+            air[i] = 1
+        if i < ground_actions and thr == b and float(yaw != 0) == hb:
+            air[i] = 1
+    return ground, air, boost, jump
+
+
+def action_mask(c, masks):
+    """DefaultAction::GetActionMask for car record c."""
+    ground, air, boost, jump = masks
+    out = list(ground if c.is_on_ground else air)
+    if c.boost == 0:
+        out = [m & (1 - bm) for m, bm in zip(out, boost)]
+    turtled = bool(c.world_contact) and F(c.world_contact_normal.z) > F(0.9)
+    if has_flip_or_jump(c) or turtled:
+        out = [m | jm for m, jm in zip(out, jump)]
+    return out

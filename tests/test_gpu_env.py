@@ -151,3 +151,49 @@ def test_auto_reset_after_goal():
         assert ln[2 + i] == 127
         np.testing.assert_allclose(obs[off[2 + i]:off[2 + i] + 127], want, atol=2e-6)
     b.Destroy()
+
+
+def test_action_masks_match_spec(torch_cuda):
+    """DefaultAction masks written by the env step equal the restatement's
+    GetActionMask on the same post-step car records (grounded and airborne
+    cars, empty tanks, cars on their roofs)."""
+    import torch
+    from emy_s_framework_amd import workloads as WL
+    n = 96
+    b = E.RocketSimBatch(n, [1, 2, 3] * (n // 3), seed=5)
+    WL.random_field_states(b, 5)
+    cars = b.GetAllCarStates()
+    for k in range(0, b.num_cars, 3):
+        cars[k].boost = 0.0
+    for k in range(1, b.num_cars, 5):  # airborne
+        cars[k].pos.z = 600.0
+        cars[k].is_on_ground = 0
+    for k in range(2, b.num_cars, 7):  # on its roof
+        cars[k].pos.z = 40.0
+        cars[k].rot.up.z, cars[k].rot.right.x, cars[k].rot.right.y = -1.0, -cars[k].rot.right.x, -cars[k].rot.right.y
+        cars[k].is_on_ground = 0
+    b.SetAllCarStates(cars)
+    lut = _lut()
+    masks = S.default_action_masks(lut)
+    b.SetActionMasks(True)
+    g = np.random.default_rng(9)
+    seen_roof = False
+    for step in range(4):
+        d = torch.tensor(g.integers(0, 90, b.num_cars).astype(np.int32), device="cuda")
+        b.StepEnv(d.data_ptr())
+        got = b.CopyBuiltDefaultActionMasks()
+        recs = b.GetAllCarStates()
+        for k in range(b.num_cars):
+            want = S.action_mask(recs[k], masks)
+            assert list(got[k]) == want, (step, k)
+            seen_roof |= bool(recs[k].world_contact and recs[k].world_contact_normal.z > 0.9)
+    b.BuildObs()
+    got = b.CopyBuiltDefaultActionMasks()
+    recs = b.GetAllCarStates()
+    assert all(list(got[k]) == S.action_mask(recs[k], masks) for k in range(b.num_cars))
+    assert seen_roof
+    b.SetActionMasks(False)
+    b.UploadDiscreteActionLut((E.CarControls * 40)(*E.default_action_lut()[:40]))
+    with pytest.raises(E.RsxError):
+        b.SetActionMasks(True)
+    b.Destroy()
