@@ -153,7 +153,7 @@ def main():
 
     import torch
     eng = _import_package()
-    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf
+    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf, margins_for
     from emy_s_framework_amd.workloads import mixed_team_sizes, random_field_states, stress_states
 
     torch.cuda.init()
@@ -182,7 +182,7 @@ def main():
     margin = None
     if use_cull:
         net = NeuralSdf(batch, seed=0)
-        margin = float(net.load(WEIGHTS_PATH)["margin_uu"])
+        margin = margins_for(net.load(WEIGHTS_PATH), args.cull_every)
 
     d_act = torch.empty(n_cars, dtype=torch.int32, device=dev)  # the captured graph's action buffer
 
@@ -242,14 +242,15 @@ def main():
     exact = run("exact collision", args.steps, args.warmup)
     cull = audit = None
     if use_cull:
-        net.attach(margin, every=args.cull_every, audit=False)
+        net.attach(margin[0], every=args.cull_every, audit=False, margin_car_uu=margin[1])
         cull = run("learned cull", args.steps, args.warmup)
-        net.attach(margin, every=args.cull_every, audit=True)
+        net.attach(margin[0], every=args.cull_every, audit=True, margin_car_uu=margin[1])
         net.reset_stats()
         run("learned cull, audited", args.audit_steps, 0, retune=False)
         audit = net.stats()
         body_ticks = args.audit_steps * args.tick_skip * (args.arenas + n_cars)
-        audit.update({"steps": args.audit_steps, "margin_uu": margin, "every_ticks": args.cull_every,
+        audit.update({"steps": args.audit_steps, "margin_ball_uu": margin[0], "margin_car_uu": margin[1],
+                      "every_ticks": args.cull_every,
                       "culled_fraction": audit["culled"] / body_ticks})
         net.detach()
     head = cull if cull is not None else exact
@@ -280,7 +281,8 @@ def main():
                                f"tick-skip {args.tick_skip}, fused env step (physics + DefaultObs + 4 stub rewards "
                                f"+ terminals + auto-reset)" + (", replayed from a hipGraph" if args.graph else ""),
                    "arenas": args.arenas, "agents": n_cars, "tick_skip": args.tick_skip,
-                   "cull": (f"learned SDF cull every {args.cull_every} ticks, margin {margin:.1f} uu, exact "
+                   "cull": (f"learned SDF cull every {args.cull_every} ticks, margins {margin[0]:.1f} uu (ball) / "
+                            f"{margin[1]:.1f} uu (cars), exact "
                             f"fallback" if cull is not None else "none (exact collision)")},
         "exact": exact,
         "cull": cull,

@@ -75,7 +75,7 @@ _SIGS = [
     ("rsx_sdf_gradients", _I, [_VP, _VP, _VP, _I, C.c_float, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("rsx_sdf_get_params", _I, [_VP, C.POINTER(C.c_float)]),
     ("rsx_sdf_set_params", _I, [_VP, C.POINTER(C.c_float)]),
-    ("rsx_attach_cull", _I, [_VP, _VP, C.c_float, _I, _I]),
+    ("rsx_attach_cull", _I, [_VP, _VP, C.c_float, C.c_float, _I, _I]),
     ("rsx_get_cull_stats", _I, [_VP, C.POINTER(C.c_uint64 * 3)]),
     ("rsx_reset_cull_stats", _I, [_VP]),
     ("rsx_cull_reach", _I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),

@@ -68,13 +68,14 @@ struct Net {
 
 // Cull configuration handed to the tick kernels.  The network runs every
 // `every` ticks; a prediction made k ticks ago still culls when it exceeds
-// margin + reach + k * (the most a body can move in one tick).
+// the body's margin + reach + k * (the most it can move in one tick).
 struct Cull {
     Net net;
     int enabled;
     int audit;              // also run the exact check on every culled body
     int every;              // ticks between network evaluations (>= 1)
-    float margin_uu;        // safety margin subtracted from the prediction
+    float margin_ball_uu;   // safety margins subtracted from the prediction
+    float margin_car_uu;
     float reach_ball_uu;    // ball: radius + gather pad + breaking threshold
     float reach_car_uu;     // car: box + wheel rays around the box centre
     float step_ball_uu;     // bound on the ball's motion in one tick

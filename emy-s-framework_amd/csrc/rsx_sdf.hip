@@ -525,18 +525,21 @@ rsx_status rsx_sdf_set_params(rsx_sdf* s, const float* h_params) {
     return RSX_OK;
 }
 
-rsx_status rsx_attach_cull(rsx_engine* e, rsx_sdf* s, float margin_uu, int32_t every_ticks, int32_t audit) {
+rsx_status rsx_attach_cull(rsx_engine* e, rsx_sdf* s, float margin_ball_uu, float margin_car_uu, int32_t every_ticks,
+                           int32_t audit) {
     if (!e) return record_failure(RSX_INVALID_ARG, "null engine");
     Cull c{};
     if (s) {
         if (s->eng != e) return record_failure(RSX_INVALID_ARG, "the network was created for another engine");
-        if (!(margin_uu >= 0.f)) return record_failure(RSX_INVALID_ARG, "margin must be >= 0");
+        if (!(margin_ball_uu >= 0.f) || !(margin_car_uu >= 0.f))
+            return record_failure(RSX_INVALID_ARG, "margins must be >= 0");
         if (every_ticks < 1) return record_failure(RSX_INVALID_ARG, "every_ticks must be >= 1");
         c.net = net_view(s);
         c.enabled = 1;
         c.audit = audit ? 1 : 0;
         c.every = every_ticks;
-        c.margin_uu = margin_uu;
+        c.margin_ball_uu = margin_ball_uu;
+        c.margin_car_uu = margin_car_uu;
         c.reach_ball_uu = s->reach_ball;
         c.reach_car_uu = s->reach_car;
         // motion bounds per tick: the ball's speed is clamped to 6000 uu/s

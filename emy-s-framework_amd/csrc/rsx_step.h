@@ -735,7 +735,8 @@ __device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAX
         }
         const float reach = b == 0 ? C.reach_ball_uu : C.reach_car_uu;
         const float step = b == 0 ? C.step_ball_uu : C.step_car_uu;
-        const bool cull = have && (W.cull_pred[b] - C.margin_uu > reach + step * (float)W.cull_age);
+        const float margin = b == 0 ? C.margin_ball_uu : C.margin_car_uu;
+        const bool cull = have && (W.cull_pred[b] - margin > reach + step * (float)W.cull_age);
         far[b] = cull;
         const bool defect = cull && C.audit && mesh_within(T.mv, q, reach * kUU2BT);
         nc += __popcll(__ballot(cull && T.own));

@@ -66,23 +66,36 @@ def main():
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--cull-steps", type=int, default=1000)
+    ap.add_argument("--remargin", default=None, metavar="WEIGHTS",
+                    help="keep these weights; recompute the margins and cull rates on the held-out split")
     args = ap.parse_args()
     import torch
     eng = _package()
-    from emy_s_framework_amd.sdf import NeuralSdf, cull_rates, train
+    from emy_s_framework_amd.sdf import NeuralSdf, cull_rates, sample_points, train, validate
     torch.cuda.init()
     host = eng.RocketSimBatch(1, 1)
     net = NeuralSdf(host, seed=args.seed)
     t0 = time.perf_counter()
-    rep = train(net, n_train=args.train, n_val=args.val, batch=args.batch, epochs=args.epochs, seed=args.seed,
-                log=lambda m: print(m, flush=True))
+    if args.remargin:
+        old = net.load(args.remargin)
+        pts = sample_points(net.mesh_triangles(), args.train + args.val, args.seed)[args.train:]
+        xva = torch.from_numpy(pts).cuda()
+        yva = torch.cat([net.exact_distance(xva[s:s + (1 << 20)]) for s in range(0, xva.shape[0], 1 << 20)])
+        rep = validate(net, xva, yva)
+        rep["weights_from"] = os.path.basename(args.remargin)
+        rep.update({k: v for k, v in old.items() if k not in rep and np.ndim(v) == 0})
+    else:
+        rep = train(net, n_train=args.train, n_val=args.val, batch=args.batch, epochs=args.epochs, seed=args.seed,
+                    log=lambda m: print(m, flush=True))
     rep["wall_seconds"] = time.perf_counter() - t0
     rb, rc = net.reach()
     rep["reach_ball_uu"], rep["reach_car_uu"] = rb, rc
     pts = kickoff_query_points(eng, steps=args.cull_steps)
-    rep["cull_rate_kickoff"] = cull_rates(net, pts, rep["margin_uu"])
+    rep["cull_rate_kickoff"] = {f"every_{e}": cull_rates(net, pts, mb, mc) for e, mb, mc in
+                                zip(rep["schedules"], rep["margin_ball_uu"], rep["margin_car_uu"])}
     net.save(args.out, margin_uu=rep["margin_uu"], max_abs_err_uu=rep["max_abs_err_uu"],
-             p99_abs_err_uu=rep["p99_abs_err_uu"])
+             p99_abs_err_uu=rep["p99_abs_err_uu"], schedules=np.array(rep["schedules"]),
+             margin_ball_uu=np.array(rep["margin_ball_uu"]), margin_car_uu=np.array(rep["margin_car_uu"]))
     print(json.dumps(rep), flush=True)
     if args.report:
         with open(args.report, "w") as f:

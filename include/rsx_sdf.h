@@ -73,13 +73,14 @@ rsx_status rsx_sdf_get_params(rsx_sdf* s, float* h_params);
 rsx_status rsx_sdf_set_params(rsx_sdf* s, const float* h_params);
 
 /* Attach the network to the engine's tick as a cull with the given safety
- * margin (uu).  The network runs every `every_ticks` ticks (>= 1); between
- * runs a body stays culled only while its prediction still exceeds margin +
- * reach + the distance it can have moved.  audit != 0 also checks every
- * culled body against the exact mesh distance each tick and counts defects.
- * s == NULL detaches (exact collision only).  The network must outlive its
- * attachment. */
-rsx_status rsx_attach_cull(rsx_engine* e, rsx_sdf* s, float margin_uu, int32_t every_ticks, int32_t audit);
+ * margins (uu) for the ball and for cars.  The network runs every
+ * `every_ticks` ticks (>= 1); between runs a body stays culled only while its
+ * prediction still exceeds its margin + reach + the distance it can have
+ * moved.  audit != 0 also checks every culled body against the exact mesh
+ * distance each tick and counts defects.  s == NULL detaches (exact
+ * collision only).  The network must outlive its attachment. */
+rsx_status rsx_attach_cull(rsx_engine* e, rsx_sdf* s, float margin_ball_uu, float margin_car_uu, int32_t every_ticks,
+                           int32_t audit);
 rsx_status rsx_get_cull_stats(rsx_sdf* s, rsx_cull_stats* out);
 rsx_status rsx_reset_cull_stats(rsx_sdf* s);
 

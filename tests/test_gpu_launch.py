@@ -163,7 +163,7 @@ def test_cull_with_multiwave_workgroups(torch_cuda):
     """The cull's network runs per wavefront; with several wavefronts per
     workgroup each uses its own LDS and waits for no other."""
     torch = torch_cuda
-    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf
+    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf, margins_for
     if not os.path.exists(WEIGHTS_PATH):
         pytest.skip("no trained weights committed")
     n = 300
@@ -175,7 +175,8 @@ def test_cull_with_multiwave_workgroups(torch_cuda):
         b = _stress_batch(n)
         s = NeuralSdf(b, seed=0)
         meta = s.load(WEIGHTS_PATH)
-        s.attach(float(meta["margin_uu"]), every=2, audit=True)
+        mb, mc = margins_for(meta, 2)
+        s.attach(mb, every=2, audit=True, margin_car_uu=mc)
         got = _run(b, acts, shape)
         st = s.stats()
         s.detach()

@@ -144,7 +144,7 @@ def test_short_training_reduces_error(net, torch_cuda):
 
 def test_cull_in_tick_is_exact_and_audited(torch_cuda):
     eng = R.import_engine()
-    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf
+    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf, margins_for
     if not os.path.exists(WEIGHTS_PATH):
         pytest.skip("no trained weights committed")
     cases = list(SC.cases().values())
@@ -156,7 +156,8 @@ def test_cull_in_tick_is_exact_and_audited(torch_cuda):
     b = eng.RocketSimBatch(len(cases), 1, seed=3)
     s = NeuralSdf(b, seed=0)
     meta = s.load(WEIGHTS_PATH)
-    s.attach(float(meta["margin_uu"]), every=2, audit=True)
+    mb, mc = margins_for(meta, 2)
+    s.attach(mb, every=2, audit=True, margin_car_uu=mc)
     culled = run_engine(b, cases)
     st = s.stats()
     s.detach()
