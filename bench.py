@@ -145,7 +145,18 @@ def main():
     ap.add_argument("--cull-every", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-cal", action="store_true",
+                    help="only copy 1 GiB device to device three times (scales the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_cal:
+        import torch
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 1 GiB x3", flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the arenas are independent and every GPU box here has one GPU "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)

@@ -57,11 +57,13 @@ __device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs&
     tick_ctx(T, G, lds_wave, la.own);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
+    W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
     uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    cull_flush(T, W);
     if (!la.own) return;
     pack_arena<MAXC>(col, A);
     col.set(Layout<MAXC>::kArenaBase + kArFlags, flags);
@@ -103,11 +105,13 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     tick_ctx(T, G, lds_wave, la.own);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
+    W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
     uint32_t flags = col.get(L::kArenaBase + kArFlags);
     for (int t = 0; t < V.ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    cull_flush(T, W);
     if (!la.own) return;
     const EnvConfig& E = V.E;
     const bool goal = ball_scored_now(A.ball);

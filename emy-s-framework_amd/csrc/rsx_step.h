@@ -59,6 +59,7 @@ struct TickScratch {
     gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
     float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
     int cull_age;                        // ticks since the network last ran (uniform over the wavefront)
+    unsigned long long cull_counts[3];   // the wavefront's queries, culled body-ticks, audit defects
 };
 
 template <int MAXC>
@@ -743,11 +744,22 @@ __device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAX
         nd += __popcll(__ballot(defect && T.own));
     }
     W.cull_age++;
-    if ((threadIdx.x & 63) == 0) {
-        if (nq) atomicAdd(&C.stats[0], nq);
-        if (nc) atomicAdd(&C.stats[1], nc);
-        if (nd) atomicAdd(&C.stats[2], nd);
-    }
+    W.cull_counts[0] += nq;
+    W.cull_counts[1] += nc;
+    W.cull_counts[2] += nd;
+}
+
+#endif
+
+#if defined(__HIPCC__)
+// Adds the wavefront's cull counters to the network's totals, once per
+// launch (per-tick atomics from every wavefront would contend on three
+// addresses).  Every lane of the wavefront must call it.
+template <int MAXC>
+__device__ void cull_flush(const TickCtx& T, const TickScratch<MAXC>& W) {
+    if (!T.cull.enabled || (threadIdx.x & 63) != 0) return;
+    for (int k = 0; k < 3; k++)
+        if (W.cull_counts[k]) atomicAdd(&T.cull.stats[k], W.cull_counts[k]);
 }
 #endif
 
