@@ -139,7 +139,8 @@ def main():
                     help="replay each env step from a captured hipGraph")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="pick the launch shape with the library's autotuner first")
-    ap.add_argument("--tuning", default=None, help="fixed launch shape 'block,arenas_per_wave' (skips the autotuner)")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shape 'block,arenas_per_wave[,pool]' (skips the autotuner)")
     ap.add_argument("--cull", choices=("auto", "on", "off"), default="auto")
     ap.add_argument("--audit-steps", type=int, default=8)
     ap.add_argument("--cull-every", type=int, default=8)
@@ -207,7 +208,7 @@ def main():
         apply_states(batch, seed=4)
         d_act.copy_(actions[0])
         res = batch.Autotune(d_act.data_ptr(), env_steps=2)
-        return [{"block": b, "arenas_per_wave": a, "ms": ms} for b, a, ms in res]
+        return [{"block": b, "arenas_per_wave": a, "pool": p, "ms": ms} for b, a, p, ms in res]
 
     def run(tag, steps, warmup, retune=True):
         """Fresh workload states, warmup, then `steps` timed fused env steps
@@ -248,7 +249,8 @@ def main():
         shape = batch.GetTuning()
         return {"value": args.arenas * steps / wall, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kern,
                 "agent_steps_per_s": n_cars * steps / wall,
-                "launch_shape": {"block": shape[0], "arenas_per_wave": shape[1]}, "autotune": tuning}
+                "launch_shape": {"block": shape[0], "arenas_per_wave": shape[1], "pool_narrowphase": shape[2]},
+                "autotune": tuning}
 
     exact = run("exact collision", args.steps, args.warmup)
     cull = audit = None

@@ -309,23 +309,24 @@ class RocketSimBatch:
     def Synchronize(self):
         _check(self._lib.rsx_synchronize(self._h))
 
-    def SetTuning(self, block_lanes=64, arenas_per_wave=64):
+    def SetTuning(self, block_lanes=64, arenas_per_wave=64, pool_narrowphase=1):
         """Launch shape of the arena kernels (results do not depend on it)."""
-        _check(self._lib.rsx_set_tuning(self._h, C.byref(Tuning(block_lanes, arenas_per_wave))))
+        _check(self._lib.rsx_set_tuning(self._h, C.byref(Tuning(block_lanes, arenas_per_wave, pool_narrowphase))))
 
     def GetTuning(self):
+        """(block_lanes, arenas_per_wave, pool_narrowphase)."""
         t = Tuning()
         _check(self._lib.rsx_get_tuning(self._h, C.byref(t)))
-        return t.block_lanes, t.arenas_per_wave
+        return t.block_lanes, t.arenas_per_wave, t.pool_narrowphase
 
     def Autotune(self, device_actions_ptr, env_steps=3):
         """Times every launch shape on the current arenas, keeps the fastest and
-        restores the arenas; returns [(block_lanes, arenas_per_wave, ms or None)]."""
-        res = (TuneResult * 16)()
+        restores the arenas; returns [(block_lanes, arenas_per_wave, pool_narrowphase, ms or None)]."""
+        res = (TuneResult * 32)()
         n = _I()
-        _check(self._lib.rsx_autotune(self._h, device_actions_ptr, env_steps, res, 16, C.byref(n)))
-        return [(r.tuning.block_lanes, r.tuning.arenas_per_wave, r.ms_per_env_step if r.feasible else None)
-                for r in res[:n.value]]
+        _check(self._lib.rsx_autotune(self._h, device_actions_ptr, env_steps, res, 32, C.byref(n)))
+        return [(r.tuning.block_lanes, r.tuning.arenas_per_wave, r.tuning.pool_narrowphase,
+                 r.ms_per_env_step if r.feasible else None) for r in res[:n.value]]
 
     def CaptureEnvGraph(self, device_actions_ptr, env_steps=1):
         """Records env_steps StepEnv calls on device_actions_ptr into a hipGraph."""

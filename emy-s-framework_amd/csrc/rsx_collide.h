@@ -46,6 +46,7 @@ struct TickCtx {
     sdf::Cull cull;    // learned broadphase cull (cull.enabled == 0: exact only)
     uint16_t* lds_act; // the wavefront's LDS activation buffer for the cull's network
     bool own;          // this lane stores its arena (false: it mirrors another lane, or pads the grid)
+    uint8_t* pool;     // the wavefront's LDS for pooled box-triangle queries (null: each lane runs its own)
 };
 
 // --- manifold bookkeeping ----------------------------------------------------

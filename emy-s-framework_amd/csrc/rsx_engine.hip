@@ -232,12 +232,14 @@ BucketArgs bucket_args(const Bucket& b) {
     return a;
 }
 
-WorldArgs world_args(const rsx_engine* e) { return WorldArgs{e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->cull}; }
+WorldArgs world_args(const rsx_engine* e, const LaunchShape& s) {
+    return WorldArgs{e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->cull, s.pool};
+}
 
 // Checks that the shape's LDS fits a workgroup and raises the kernels'
 // dynamic-LDS limit when it needs more than 64 KiB.
 rsx_status prepare_shape(rsx_engine* e, const LaunchShape& s) {
-    const size_t lds = arena_lds_per_wave(e->cull.enabled != 0) * (s.block / 64);
+    const size_t lds = arena_lds_per_wave(e->cull.enabled != 0, s.pool != 0) * (s.block / 64);
     if (lds > e->lds_per_block_max)
         return fail(RSX_INVALID_ARG, "launch shape needs " + std::to_string(lds) + " bytes of LDS per workgroup; the "
                                          "device allows " + std::to_string(e->lds_per_block_max));
@@ -248,7 +250,7 @@ rsx_status prepare_shape(rsx_engine* e, const LaunchShape& s) {
 // One fused env step of every arena: one launch (the mixed kernel when team
 // sizes differ).
 hipError_t env_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, hipStream_t st) {
-    const WorldArgs w = world_args(e);
+    const WorldArgs w = world_args(e, s);
     EnvArgs v;
     v.act = act;
     v.lut = e->d_lut;
@@ -1010,7 +1012,7 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
     if (ticks < 0) return fail(RSX_INVALID_ARG, "ticks must be >= 0");
     HIP_TRY(hipSetDevice(e->device));
     RSX_TRY(prepare_shape(e, e->shape));
-    const WorldArgs w = world_args(e);
+    const WorldArgs w = world_args(e, e->shape);
     for (int k = 0; k < 3; k++) {
         const Bucket& b = e->bucket[k];
         if (b.n) HIP_TRY(launch_ticks(b.maxc, bucket_args(b), w, ticks, e->shape, (hipStream_t)stream));
@@ -1021,9 +1023,10 @@ rsx_status rsx_step_ticks(rsx_engine* e, int32_t ticks, void* stream) {
 rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t) {
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (!t) return fail(RSX_INVALID_ARG, "null tuning");
-    const LaunchShape s{t->block_lanes, t->arenas_per_wave};
+    const LaunchShape s{t->block_lanes, t->arenas_per_wave, t->pool_narrowphase};
     if (!shape_valid(s))
-        return fail(RSX_INVALID_ARG, "block_lanes must be 64, 128 or 256 and arenas_per_wave 64, 32, 16 or 8");
+        return fail(RSX_INVALID_ARG,
+                    "block_lanes must be 64, 128 or 256, arenas_per_wave 64, 32, 16 or 8, pool_narrowphase 0 or 1");
     HIP_TRY(hipSetDevice(e->device));
     RSX_TRY(prepare_shape(e, s));
     e->shape = s;
@@ -1036,6 +1039,7 @@ rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t) {
     if (!t) return fail(RSX_INVALID_ARG, "null tuning");
     t->block_lanes = e->shape.block;
     t->arenas_per_wave = e->shape.arenas_per_wave;
+    t->pool_narrowphase = e->shape.pool;
     return RSX_OK;
 }
 
@@ -1047,9 +1051,10 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
     static const int kBlocks[] = {64, 128, 256};
     static const int kApw[] = {64, 32, 16, 8};
     std::vector<rsx_tune_result> res;
-    for (int bl : kBlocks)
-        for (int apw : kApw) res.push_back(rsx_tune_result{{bl, apw}, -1.f, 0});
-    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 12");
+    for (int pool = 1; pool >= 0; pool--)
+        for (int bl : kBlocks)
+            for (int apw : kApw) res.push_back(rsx_tune_result{{bl, apw, pool}, -1.f, 0});
+    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 24");
     // Snapshot of everything a trial overwrites, restored after each trial
     // and at the end.
     std::vector<std::pair<void*, size_t>> bufs;
@@ -1087,8 +1092,9 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
         }
         int best = -1;
         for (size_t k = 0; k < res.size(); k++) {
-            const LaunchShape s{res[k].tuning.block_lanes, res[k].tuning.arenas_per_wave};
-            const size_t lds = arena_lds_per_wave(e->cull.enabled != 0) * (s.block / 64);
+            const LaunchShape s{res[k].tuning.block_lanes, res[k].tuning.arenas_per_wave,
+                                res[k].tuning.pool_narrowphase};
+            const size_t lds = arena_lds_per_wave(e->cull.enabled != 0, s.pool != 0) * (s.block / 64);
             if (lds > e->lds_per_block_max || prepare_arena_kernels(lds) != hipSuccess) {
                 (void)hipGetLastError();
                 continue;  // does not fit the workgroup's LDS
@@ -1109,7 +1115,8 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
         HIP_TRY(restore());
         HIP_TRY(hipStreamSynchronize(st));
         if (best < 0) return fail(RSX_HIP_ERROR, "no launch shape fits this device");
-        e->shape = LaunchShape{res[best].tuning.block_lanes, res[best].tuning.arenas_per_wave};
+        e->shape = LaunchShape{res[best].tuning.block_lanes, res[best].tuning.arenas_per_wave,
+                               res[best].tuning.pool_narrowphase};
         e->gen++;
         return RSX_OK;
     };

@@ -17,7 +17,7 @@ size_t g_lds_done[3] = {65536, 65536, 65536};
 hipError_t launch_env(int maxc, const BucketArgs& b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
                       hipStream_t st) {
     if (!b.n) return hipSuccess;
-    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0);
+    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0, w.pool != 0);
     const size_t lds = per_wave * (s.block / 64);
     dim3 grid(blocks_for_waves(arena_waves(b.n, s), s)), block(s.block);
     if (maxc == 2)

@@ -31,7 +31,7 @@ hipError_t launch_env_mixed(const BucketArgs* b, const WorldArgs& w, const EnvAr
                             hipStream_t st) {
     const int w2 = arena_waves(b[0].n, s), w4 = arena_waves(b[1].n, s), w6 = arena_waves(b[2].n, s);
     if (!(w2 + w4 + w6)) return hipSuccess;
-    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0);
+    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0, w.pool != 0);
     const size_t lds = per_wave * (s.block / 64);
     dim3 grid(blocks_for_waves(w2 + w4 + w6, s)), block(s.block);
     hipLaunchKernelGGL(k_env_mixed, grid, block, lds, st, b[0], b[1], b[2], w6, w4, w, v, s.arenas_per_wave,

@@ -17,7 +17,7 @@ size_t g_lds_done[3] = {65536, 65536, 65536};
 hipError_t launch_ticks(int maxc, const BucketArgs& b, const WorldArgs& w, int ticks, const LaunchShape& s,
                         hipStream_t st) {
     if (!b.n) return hipSuccess;
-    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0);
+    const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0, w.pool != 0);
     const size_t lds = per_wave * (s.block / 64);
     dim3 grid(blocks_for_waves(arena_waves(b.n, s), s)), block(s.block);
     if (maxc == 2)
@@ -44,18 +44,20 @@ hipError_t prepare_arena_kernels(size_t bytes) {
 }
 
 // Shape arithmetic shared by every launcher.
-size_t arena_lds_per_wave(bool cull) {
-    size_t b = sizeof(gjk::Simplex) * 64;
+size_t arena_lds_per_wave(bool cull, bool pool) {
+    size_t b = (size_t)kSimplexBytes + (pool ? (size_t)kPoolBytes : 0);
     if (cull && (size_t)sdf::kLdsActBytes > b) b = sdf::kLdsActBytes;
     return (b + 15) & ~(size_t)15;
 }
+
+size_t arena_pool_offset() { return kPoolOffset; }
 
 int arena_waves(int n, const LaunchShape& s) { return (n + s.arenas_per_wave - 1) / s.arenas_per_wave; }
 
 bool shape_valid(const LaunchShape& s) {
     const bool block_ok = s.block == 64 || s.block == 128 || s.block == 256;
     const int a = s.arenas_per_wave;
-    return block_ok && (a == 64 || a == 32 || a == 16 || a == 8);
+    return block_ok && (a == 64 || a == 32 || a == 16 || a == 8) && (s.pool == 0 || s.pool == 1);
 }
 
 }  // namespace rsx

@@ -50,6 +50,7 @@ struct WorldArgs {
     MeshView mv;
     const PadGeom* pads;
     sdf::Cull cull;
+    int pool;                  // pool each wavefront's box-triangle queries (car_gjk_pool)
 };
 
 // The env half of a fused env step.
@@ -69,11 +70,15 @@ struct EnvArgs {
 struct LaunchShape {
     int block = 64;
     int arenas_per_wave = 64;
+    int pool = 1;              // pooled box-triangle queries
 };
 
-// LDS bytes one wavefront needs: the lanes' GJK / convex-cast simplices, or
-// (sharing the space) the cull's activation buffer when the cull is on.
-size_t arena_lds_per_wave(bool cull);
+// LDS bytes one wavefront needs: the lanes' GJK / convex-cast simplices and
+// (with pooling) the pooled queries, or -- sharing the space, at another time
+// of the tick -- the cull's activation buffer when the cull is on.
+size_t arena_lds_per_wave(bool cull, bool pool);
+// Offset of the pooled queries' LDS within a wavefront's share.
+size_t arena_pool_offset();
 // Wavefronts of a launch over n arenas.
 int arena_waves(int n, const LaunchShape& s);
 // Whether s is one of the supported shapes.

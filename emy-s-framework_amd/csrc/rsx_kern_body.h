@@ -10,6 +10,10 @@
 
 namespace rsx {
 
+// A wavefront's LDS: its lanes' simplices, then the pooled queries.
+constexpr int kSimplexBytes = (64 * (int)sizeof(gjk::Simplex) + 15) & ~15;
+constexpr int kPoolOffset = kSimplexBytes;
+
 // The lane's arena within a wavefront whose first arena is `first`:
 // lanes past arenas_per_wave mirror lane (l % arenas_per_wave); lanes past
 // the bucket's end run its last arena.  Neither stores.
@@ -24,7 +28,7 @@ __device__ inline LaneArena lane_arena(int first, int n, int apw) {
     return LaneArena{a0 < n ? a0 : n - 1, l < apw && a0 < n};
 }
 
-__device__ inline void tick_ctx(TickCtx& T, const WorldArgs& G, uint8_t* lds_wave, bool own) {
+__device__ inline void tick_ctx(TickCtx& T, const WorldArgs& G, uint8_t* lds_wave, bool own, int pool_off) {
     T.cg = G.cg;
     T.bg = G.bg;
     T.mv = G.mv;
@@ -34,6 +38,7 @@ __device__ inline void tick_ctx(TickCtx& T, const WorldArgs& G, uint8_t* lds_wav
     T.lds_act = (uint16_t*)lds_wave;
     T.cull = G.cull;
     T.own = own;
+    T.pool = G.pool ? lds_wave + pool_off : nullptr;
 }
 
 // `ticks` ticks of the bucket's arenas under their stored controls
@@ -54,7 +59,7 @@ __device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs&
                                      c.handbrake != 0};
     }
     TickCtx T;
-    tick_ctx(T, G, lds_wave, la.own);
+    tick_ctx(T, G, lds_wave, la.own, kPoolOffset);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
@@ -102,7 +107,7 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
         prev[i][7] = c.handbrake ? 1.f : 0.f;
     }
     TickCtx T;
-    tick_ctx(T, G, lds_wave, la.own);
+    tick_ctx(T, G, lds_wave, la.own, kPoolOffset);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;

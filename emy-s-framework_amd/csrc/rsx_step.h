@@ -56,6 +56,8 @@ struct TickScratch {
     Solver<MAXC> S;
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
     int cand[48];
+    int8_t pidx[48];                     // candidate -> index among the lane's pooled queries, -1: separated
+    int pool_base;                       // the lane's first slot in the wavefront's pooled results
     gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
     float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
     int cull_age;                        // ticks since the network last ran (uniform over the wavefront)
@@ -201,9 +203,114 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
 // (btConvexConcaveCollisionAlgorithm with the box as the convex body: the
 // triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
 // one manifold per car and mesh, refreshed after all triangles).
+// Early out of a box-triangle query: the box (world transform `child`,
+// half extents with margin `hm`) lies wholly on one side of the triangle's
+// plane, farther than `breaking` from it.
+RSX_HD bool box_tri_separated(const Xf& child, const M3& inv_basis, V3 hm, const V3* tri, float breaking) {
+    V3 n = normalized(cross(tri[1] - tri[0], tri[2] - tri[0]));
+    for (int side = 0; side < 2; side++) {
+        V3 lp = gjk::box_support(hm, inv_basis * n);
+        V3 wp = xform(child, lp);
+        float dist = dot(n, tri[0]) - dot(n, wp);
+        if (dist > breaking) return true;
+        n = n * -1.f;
+    }
+    return false;
+}
+
+// Result of one pooled box-triangle query (gjk::closest_points).
+struct PooledContact {
+    V3 normal_on_b, point_on_b;
+    float depth;
+    int hit;
+};
+
+// LDS of a wavefront's pooled box-triangle queries: the 64 lanes' car child
+// transforms, item counts, the item list (owner lane, triangle) and results.
+// Queries past kPoolCap run in their owner lane.
+constexpr int kPoolCap = 448;
+constexpr int kPoolXfOff = 0;
+constexpr int kPoolCntOff = kPoolXfOff + 64 * (int)sizeof(Xf);
+constexpr int kPoolItemOff = kPoolCntOff + 64 * 4;
+constexpr int kPoolResOff = kPoolItemOff + kPoolCap * 4;
+constexpr int kPoolBytes = kPoolResOff + kPoolCap * (int)sizeof(PooledContact);
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Box-triangle queries of car `ci`'s candidates on every lane of the
+// wavefront, spread over its 64 lanes: each lane filters its candidates
+// (meshes outside `mask`, box_tri_separated), the wavefront lists the
+// survivors in LDS, every lane runs entries lane, lane + 64, ... with its own
+// GJK working set, and the results wait in LDS for car_vs_mesh.  A lane's
+// queries otherwise run one after another while the rest of its wavefront
+// idles.  Every lane of the wavefront must call it.
 template <int MAXC>
-RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifold& m, int ci, int mesh,
-                        const int* cand, int ncand) {
+__device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int ci, uint32_t mask,
+                             int ncand) {
+    const int l = threadIdx.x & 63;
+    const CarGeom& g = T.cg;
+    Xf* xfs = (Xf*)(T.pool + kPoolXfOff);
+    int* cnt = (int*)(T.pool + kPoolCntOff);
+    uint32_t* items = (uint32_t*)(T.pool + kPoolItemOff);
+    PooledContact* res = (PooledContact*)(T.pool + kPoolResOff);
+    Xf child = xf_identity();
+    int n_items = 0;
+    if (mask) {
+        child = car_child_xf(g, A.car[ci].xf);
+        const M3 inv_basis = inverse(child.b);
+        const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+        for (int i = 0; i < ncand; i++) {
+            V3 tri[3];
+            load_tri(T.mv, cand_tri(W.cand[i]), tri[0], tri[1], tri[2]);
+            const bool keep = ((mask >> cand_mesh(W.cand[i])) & 1u) &&
+                              !box_tri_separated(child, inv_basis, hm, tri, g.breaking);
+            W.pidx[i] = keep ? (int8_t)n_items++ : (int8_t)-1;
+        }
+    }
+    sdf::wave_sync();  // the previous readers of the pool are done
+    xfs[l] = child;
+    cnt[l] = n_items;
+    sdf::wave_sync();
+    int base = 0, total = 0;
+    for (int j = 0; j < 64; j++) {
+        const int c = cnt[j];
+        base += j < l ? c : 0;
+        total += c;
+    }
+    W.pool_base = base;
+    if (mask)
+        for (int i = 0; i < ncand; i++) {
+            const int it = base + W.pidx[i];
+            if (W.pidx[i] >= 0 && it < kPoolCap) items[it] = ((uint32_t)l << 16) | (uint32_t)cand_tri(W.cand[i]);
+        }
+    sdf::wave_sync();
+    const int n = total < kPoolCap ? total : kPoolCap;
+    const float max_d = g.box_margin + g.breaking;
+    for (int it0 = 0; it0 < n; it0 += 64) {
+        const int it = it0 + l;
+        if (it < n) {
+            const uint32_t e = items[it];
+            gjk::Shapes sh;
+            load_tri(T.mv, (int)(e & 0xffffu), sh.tri[0], sh.tri[1], sh.tri[2]);
+            sh.h = g.box_implicit;
+            sh.margin = g.box_margin;
+            gjk::Contact1 ct{};
+            PooledContact r;
+            r.hit = gjk::closest_points(W.gj, *T.sx, sh, xfs[e >> 16], xf_identity(), max_d * max_d, ct) ? 1 : 0;
+            r.normal_on_b = ct.normal_on_b;
+            r.point_on_b = ct.point_on_b;
+            r.depth = ct.depth;
+            res[it] = r;
+        }
+    }
+    sdf::wave_sync();
+}
+#endif
+
+template <int MAXC>
+RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
+                        int ncand) {
+    gjk::GjkEpa& G = W.gj;
+    const int* cand = W.cand;
     const CarGeom& g = T.cg;
     Car& c = A.car[ci];
     const Xf child = car_child_xf(g, c.xf);
@@ -211,6 +318,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifo
     const M3 inv_basis = inverse(child.b);
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
     const float max_d = g.box_margin + 0.f + m.breaking;
+    const PooledContact* pooled = T.pool ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
     for (int i = 0; i < ncand; i++) {
         const int pk = cand[i];
         if (cand_mesh(pk) < mesh) continue;
@@ -218,18 +326,20 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, gjk::GjkEpa& G, Manifo
         const int t = cand_tri(pk);
         gjk::Shapes sh;
         load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
-        // early out: the box is wholly on one side of the triangle's plane
-        {
-            V3 n = normalized(cross(sh.tri[1] - sh.tri[0], sh.tri[2] - sh.tri[0]));
-            bool skip = false;
-            for (int side = 0; side < 2 && !skip; side++) {
-                V3 lp = gjk::box_support(hm, inv_basis * n);
-                V3 wp = xform(child, lp);
-                float dist = dot(n, sh.tri[0]) - dot(n, wp);
-                if (dist > m.breaking) skip = true;
-                n = n * -1.f;
+        if (pooled) {
+            // the wavefront ran this lane's queries already (car_gjk_pool)
+            if (W.pidx[i] < 0) continue;
+            const int it = W.pool_base + W.pidx[i];
+            if (it < kPoolCap) {
+                const PooledContact r = pooled[it];
+                if (r.hit) {
+                    int idx = manifold_add(m, c.xf, trB, r.normal_on_b, r.point_on_b, r.depth, 0.6f, 0.3f);
+                    if (idx >= 0) contact_added(A, T, m, idx, t);
+                }
+                continue;
             }
-            if (skip) continue;
+        } else if (box_tri_separated(child, inv_basis, hm, sh.tri, m.breaking)) {
+            continue;
         }
         sh.h = g.box_implicit;
         sh.margin = g.box_margin;
@@ -333,14 +443,13 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         // response disabled; one demolished during this tick's collision
         // callbacks keeps both until the next tick (Car::_PreTickUpdate)
         bool demoed = (d >= 1) && !A.car[d - 1].active;
-        if (demoed) continue;  // no contact response: never solved
         const int cell = bp_cell_index(T.mv, W.bmin[d]);
         uint32_t mask = T.mv.bp_meshes[cell];
         // statics: meshes then planes.  The body's candidate triangles are
         // gathered once; a mesh with none of them cannot add a contact, and its
         // pair would only carry an empty manifold, so it is skipped.
         int ncand = 0;
-        if (mask && active[d] && !far[d]) {
+        if (!demoed && mask && active[d] && !far[d]) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
                 ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
@@ -356,6 +465,10 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         } else {
             mask = 0;
         }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (T.pool && d >= 1) car_gjk_pool(A, T, W, d - 1, mask, ncand);
+#endif
+        if (demoed) continue;  // no contact response: never solved
         for (int ms = 0; ms < kNumMeshes; ms++) {
             if (!((mask >> ms) & 1)) continue;
             const float* ma = T.mv.mesh_aabb + ms * 6;
@@ -377,7 +490,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
                 if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand);
-                else car_vs_mesh(A, T, W.gj, m, d - 1, ms, W.cand, ncand);
+                else car_vs_mesh(A, T, W, m, d - 1, ms, ncand);
             }
             mf_close(L, k);
         }

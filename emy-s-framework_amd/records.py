@@ -91,7 +91,7 @@ assert C.sizeof(ArenaInfo) == 32
 
 class Tuning(C.Structure):
     """rsx_tuning (include/rsx.h): launch shape of the arena kernels."""
-    _fields_ = [("block_lanes", C.c_int32), ("arenas_per_wave", C.c_int32)]
+    _fields_ = [("block_lanes", C.c_int32), ("arenas_per_wave", C.c_int32), ("pool_narrowphase", C.c_int32)]
 
 
 class TuneResult(C.Structure):

@@ -142,6 +142,7 @@ int cc_step(void* p, int ticks) {
     T.cull = sdf::Cull{};
     T.lds_act = nullptr;
     T.own = true;
+    T.pool = nullptr;
     int scored = 0;
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;

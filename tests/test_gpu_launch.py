@@ -58,7 +58,7 @@ def test_launch_shapes_are_bit_identical(torch_cuda):
     acts = _actions(torch, b.num_cars, 11, 3)
     want = _run(b, acts)
     b.Destroy()
-    for shape in ((128, 64), (128, 32), (256, 16), (256, 8), (64, 8)):
+    for shape in ((64, 64, 0), (128, 64, 1), (128, 32, 0), (256, 16, 1), (256, 8, 1), (64, 8, 0)):
         b = _stress_batch(n)
         got = _run(b, acts, shape)
         assert b.GetTuning() == shape
@@ -69,10 +69,10 @@ def test_launch_shapes_are_bit_identical(torch_cuda):
 
 def test_invalid_shapes_are_rejected(torch_cuda):
     b = E.RocketSimBatch(4, 1)
-    for shape in ((512, 64), (96, 64), (64, 48), (64, 0)):
+    for shape in ((512, 64, 1), (96, 64, 1), (64, 48, 1), (64, 0, 1), (64, 64, 2)):
         with pytest.raises(E.RsxError):
             b.SetTuning(*shape)
-    assert b.GetTuning() == (64, 64)
+    assert b.GetTuning() == (64, 64, 1)
     b.Destroy()
 
 
@@ -152,7 +152,7 @@ def test_mixed_launch_shapes_are_bit_identical(torch_cuda):
     acts = _actions(torch, b.num_cars, 3, 2)
     want = _run(b, acts)
     b.Destroy()
-    for shape in ((256, 16), (128, 8)):
+    for shape in ((256, 16, 1), (128, 8, 0), (64, 64, 0)):
         b = make()
         got = _run(b, acts, shape)
         b.Destroy()
@@ -171,7 +171,7 @@ def test_cull_with_multiwave_workgroups(torch_cuda):
     acts = _actions(torch, b.num_cars, 17, 3)
     want = _run(b, acts)
     b.Destroy()
-    for shape in ((64, 64), (256, 16)):
+    for shape in ((64, 64, 1), (256, 16, 1), (128, 64, 0)):
         b = _stress_batch(n)
         s = NeuralSdf(b, seed=0)
         meta = s.load(WEIGHTS_PATH)
@@ -234,11 +234,11 @@ def test_autotune_restores_state_and_keeps_the_fastest(torch_cuda):
     b.StepEnv(d.data_ptr())
     before = _snapshot(b)
     res = b.Autotune(d.data_ptr(), env_steps=1)
-    assert len(res) == 12
-    feas = [r for r in res if r[2] is not None]
-    assert feas and all(r[2] > 0 for r in feas)
-    best = min(feas, key=lambda r: r[2])
-    assert b.GetTuning() == (best[0], best[1])
+    assert len(res) == 24
+    feas = [r for r in res if r[3] is not None]
+    assert feas and all(r[3] > 0 for r in feas)
+    best = min(feas, key=lambda r: r[3])
+    assert b.GetTuning() == best[:3]
     assert _snapshot(b) == before
     b.StepEnv(d.data_ptr())
     got = _snapshot(b)
