@@ -199,10 +199,6 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
     manifold_refresh(m, b.xf, trB);
 }
 
-// Narrowphase of car `ci`'s box against mesh `mesh`
-// (btConvexConcaveCollisionAlgorithm with the box as the convex body: the
-// triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
-// one manifold per car and mesh, refreshed after all triangles).
 // Early out of a box-triangle query: the box (world transform `child`,
 // half extents with margin `hm`) lies wholly on one side of the triangle's
 // plane, farther than `breaking` from it.
@@ -306,6 +302,12 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
 }
 #endif
 
+// Narrowphase of car `ci`'s box against mesh `mesh`
+// (btConvexConcaveCollisionAlgorithm with the box as the convex body: the
+// triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
+// one manifold per car and mesh, refreshed after all triangles).  With
+// pooling the queries ran in car_gjk_pool and only their results are read
+// here, in the same order.
 template <int MAXC>
 RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
                         int ncand) {
