@@ -46,6 +46,7 @@ struct DevWorld {
     int* tri_rank = nullptr;
     int* cell_start = nullptr;
     int* cell_tris = nullptr;
+    GridEntry* cell_ent = nullptr;
     uint16_t* bp_meshes = nullptr;
     float* mesh_aabb = nullptr;
     float* tri_info = nullptr;
@@ -343,6 +344,7 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     if ((s = upload(&e->dw.tri_rank, e->hw.tri_rank)) != RSX_OK) return s;
     if ((s = upload(&e->dw.cell_start, e->hw.cell_start)) != RSX_OK) return s;
     if ((s = upload(&e->dw.cell_tris, e->hw.cell_tris)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.cell_ent, e->hw.cell_ent)) != RSX_OK) return s;
     if ((s = upload(&e->dw.bp_meshes, e->hw.bp_meshes)) != RSX_OK) return s;
     if ((s = upload(&e->dw.mesh_aabb, e->hw.mesh_aabb)) != RSX_OK) return s;
     if ((s = upload(&e->dw.tri_info, e->hw.tri_info)) != RSX_OK) return s;
@@ -354,6 +356,7 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     e->dview.tri_rank = e->dw.tri_rank;
     e->dview.cell_start = e->dw.cell_start;
     e->dview.cell_tris = e->dw.cell_tris;
+    e->dview.cell_ent = e->dw.cell_ent;
     e->dview.bp_meshes = e->dw.bp_meshes;
     e->dview.mesh_aabb = e->dw.mesh_aabb;
     e->dview.tri_info = e->dw.tri_info;
@@ -472,7 +475,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
     (void)hipFree(e->d_mask_tables);
     (void)hipFree(e->d_masks);
     (void)hipFree(e->d_idx_stage);
-    void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris,
+    void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris, e->dw.cell_ent,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
     for (void* p : ptrs) (void)hipFree(p);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);

@@ -26,6 +26,7 @@ MeshView HostWorld::view() const {
     v.tri_rank = tri_rank.data();
     v.cell_start = cell_start.data();
     v.cell_tris = cell_tris.data();
+    v.cell_ent = cell_ent.data();
     v.susp_world = nullptr;
     v.bp_meshes = bp_meshes.data();
     v.mesh_aabb = mesh_aabb.data();
@@ -383,6 +384,25 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
         w.cell_tris.insert(w.cell_tris.end(), cells[c].begin(), cells[c].end());
     }
     w.cell_start[ncell] = (int)w.cell_tris.size();
+    // gather entries (after the tree ranks are known)
+    w.cell_ent.resize(w.cell_tris.size());
+    {
+        const float inv = 1.0f / w.grid_cell;
+        for (size_t k = 0; k < w.cell_tris.size(); k++) {
+            const int t = w.cell_tris[k];
+            const float* p = &w.tri[(size_t)t * 9];
+            V3 a = V3{p[0], p[1], p[2]}, b = V3{p[3], p[4], p[5]}, c = V3{p[6], p[7], p[8]};
+            V3 mn = vmin(vmin(a, b), c), mx = vmax(vmax(a, b), c);
+            GridEntry& g = w.cell_ent[k];
+            g.key = (w.tri_mesh[t] << kCandMeshShift) | (w.tri_rank[t] << kCandRankShift) | t;
+            const int fx = clampi((int)std::floor((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
+            const int fy = clampi((int)std::floor((mn.y - w.grid_min.y) * inv), 0, w.gy - 1);
+            const int fz = clampi((int)std::floor((mn.z - w.grid_min.z) * inv), 0, w.gz - 1);
+            g.first = fx | (fy << 10) | (fz << 20);
+            g.mn[0] = mn.x, g.mn[1] = mn.y, g.mn[2] = mn.z;
+            g.mx[0] = mx.x, g.mx[1] = mx.y, g.mx[2] = mx.z;
+        }
+    }
 
     // RocketSim broadphase static cell lists (btRSBroadphase.cpp:72-136): a
     // mesh is listed in the 27 neighbours of every cell its triangles touch.

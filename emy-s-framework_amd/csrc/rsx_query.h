@@ -82,17 +82,17 @@ RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
                 int cell = (ix * w.gy + iy) * w.gz + iz;
                 int s = w.cell_start[cell], e = w.cell_start[cell + 1];
                 for (int k = s; k < e; k++) {
-                    int t = w.cell_tris[k];
-                    V3 a, b, c;
-                    load_tri(w, t, a, b, c);
-                    V3 tmn = vmin(vmin(a, b), c);
+                    const GridEntry g = w.cell_ent[k];
                     // first cell of the triangle's own range inside the query range
-                    int fx = maxi(x0, clampi((int)floorf((tmn.x - w.grid_min.x) * inv), 0, w.gx - 1));
-                    int fy = maxi(y0, clampi((int)floorf((tmn.y - w.grid_min.y) * inv), 0, w.gy - 1));
-                    int fz = maxi(z0, clampi((int)floorf((tmn.z - w.grid_min.z) * inv), 0, w.gz - 1));
+                    const int fx = maxi(x0, g.first & 1023);
+                    const int fy = maxi(y0, (g.first >> 10) & 1023);
+                    const int fz = maxi(z0, g.first >> 20);
                     if (fx != ix || fy != iy || fz != iz) continue;
-                    if (!tri_aabb_overlap(a, b, c, mn, mx)) continue;
-                    if (n < cap) out[n++] = (w.tri_mesh[t] << kCandMeshShift) | (w.tri_rank[t] << kCandRankShift) | t;
+                    // tri_aabb_overlap on the stored AABB: z, then x, then y
+                    if (g.mn[2] > mx.z || g.mx[2] < mn.z) continue;
+                    if (g.mn[0] > mx.x || g.mx[0] < mn.x) continue;
+                    if (g.mn[1] > mx.y || g.mx[1] < mn.y) continue;
+                    if (n < cap) out[n++] = g.key;
                 }
             }
     // insertion sort of the packed keys

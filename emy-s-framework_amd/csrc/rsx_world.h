@@ -45,6 +45,18 @@ struct PadGeom {
     V3 pos_uu;
 };
 
+// One triangle's entry in a uniform-grid cell, with what the candidate
+// gather tests so that a cell is scanned with one contiguous read per entry:
+// the packed candidate key (mesh, tree rank, index), the grid cell of the
+// triangle's AABB minimum (x | y << 10 | z << 20, clamped to the grid) and
+// the AABB itself (BT units).
+struct GridEntry {
+    int key;
+    int first;
+    float mn[3];
+    float mx[3];
+};
+
 // Triangle mesh of the arena plus the spatial indexes used by the queries.
 struct MeshView {
     const float* tri;          // [num_tris][9] vertices, BT units
@@ -52,6 +64,7 @@ struct MeshView {
     const int* tri_rank;       // traversal rank of the triangle inside its mesh's tree
     const int* cell_start;     // uniform grid CSR: [num_cells + 1]
     const int* cell_tris;      // triangle ids
+    const GridEntry* cell_ent; // the same lists as gather entries
     const uint8_t* susp_world; // suspension grid cells with world geometry (bled), 32*56*8
     const uint16_t* bp_meshes; // broadphase grid: per cell, mask of meshes listed in the cell
     const float* mesh_aabb;    // [16][6] shape AABB min xyz, max xyz (BT)
