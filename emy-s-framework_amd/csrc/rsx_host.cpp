@@ -384,6 +384,34 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
         w.cell_tris.insert(w.cell_tris.end(), cells[c].begin(), cells[c].end());
     }
     w.cell_start[ncell] = (int)w.cell_tris.size();
+    // the candidate lists' bound: distinct triangles listed in any 2 x 2 x 2
+    // block of cells
+    {
+        std::vector<int> mark(ntri, -1);
+        int worst = 0, stamp = 0;
+        for (int x = 0; x < w.gx; x++)
+            for (int y = 0; y < w.gy; y++)
+                for (int z = 0; z < w.gz; z++) {
+                    int cnt = 0;
+                    stamp++;
+                    for (int ix = x; ix <= std::min(x + 1, w.gx - 1); ix++)
+                        for (int iy = y; iy <= std::min(y + 1, w.gy - 1); iy++)
+                            for (int iz = z; iz <= std::min(z + 1, w.gz - 1); iz++) {
+                                const int c = (ix * w.gy + iy) * w.gz + iz;
+                                for (int k = w.cell_start[c]; k < w.cell_start[c + 1]; k++)
+                                    if (mark[w.cell_tris[k]] != stamp) {
+                                        mark[w.cell_tris[k]] = stamp;
+                                        cnt++;
+                                    }
+                            }
+                    worst = std::max(worst, cnt);
+                }
+        if (worst > kMaxCand) {
+            err = "a 2x2x2 block of grid cells lists " + std::to_string(worst) +
+                  " triangles; the candidate lists hold " + std::to_string(kMaxCand);
+            return false;
+        }
+    }
     // gather entries (after the tree ranks are known)
     w.cell_ent.resize(w.cell_tris.size());
     {

@@ -67,6 +67,13 @@ RSX_HD int cand_mesh(int p) { return p >> kCandMeshShift; }
 // tree traversal rank); see cand_tri / cand_mesh.  Returns the count (capped
 // at cap).  A triangle is listed in every grid cell its AABB touches, so it is
 // taken only from the first of those cells that the query visits.
+// Capacity of a candidate list.  Every query box is smaller than a grid cell
+// (4 BT) along each axis -- the ball's AABB 3.8 BT, a car box's at most
+// 3.1 BT, a wheel ray's well under 1 BT -- so it meets at most 2 x 2 x 2
+// cells, and build_world rejects meshes where such a block lists more
+// triangles than this (the soccar meshes: 220).
+constexpr int kMaxCand = 224;
+
 RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
     const float inv = 1.0f / w.grid_cell;
     int x0 = clampi((int)floorf((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
@@ -153,8 +160,8 @@ RSX_HD bool ray_triangle(V3 from, V3 to, V3 v0, V3 v1, V3 v2, float& hit_fractio
 // has shown to be out of their reach.
 RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit, bool with_mesh = true) {
     V3 mn = vmin(from, to), mx = vmax(from, to);
-    int cand[48];
-    int n = with_mesh ? gather_tris(w, mn, mx, cand, 48) : 0;
+    int cand[kMaxCand];
+    int n = with_mesh ? gather_tris(w, mn, mx, cand, kMaxCand) : 0;
     float frac = hit.object >= 0 ? hit.fraction : 1.0f;
     for (int i = 0; i < n; i++) {
         V3 a, b, c;

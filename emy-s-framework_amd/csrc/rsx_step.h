@@ -55,8 +55,8 @@ struct TickScratch {
     MfList L;
     Solver<MAXC> S;
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
-    int cand[48];
-    int8_t pidx[48];                     // candidate -> index among the lane's pooled queries, -1: separated
+    int cand[kMaxCand];
+    int16_t pidx[kMaxCand];              // candidate -> index among the lane's pooled queries, -1: separated
     int pool_base;                       // the lane's first slot in the wavefront's pooled results
     gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
     float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
@@ -259,7 +259,7 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
             load_tri(T.mv, cand_tri(W.cand[i]), tri[0], tri[1], tri[2]);
             const bool keep = ((mask >> cand_mesh(W.cand[i])) & 1u) &&
                               !box_tri_separated(child, inv_basis, hm, tri, g.breaking);
-            W.pidx[i] = keep ? (int8_t)n_items++ : (int8_t)-1;
+            W.pidx[i] = keep ? (int16_t)n_items++ : (int16_t)-1;
         }
     }
     sdf::wave_sync();  // the previous readers of the pool are done
@@ -454,12 +454,12 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         if (!demoed && mask && active[d] && !far[d]) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
-                ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, 48);
+                ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, kMaxCand);
             } else {
                 Xf child = car_child_xf(g, A.car[d - 1].xf);
                 V3 cmn, cmx;
                 box_child_aabb(g, child, cmn, cmx);
-                ncand = gather_tris(T.mv, cmn, cmx, W.cand, 48);
+                ncand = gather_tris(T.mv, cmn, cmx, W.cand, kMaxCand);
             }
             uint32_t cmask = 0;
             for (int i = 0; i < ncand; i++) cmask |= 1u << cand_mesh(W.cand[i]);
