@@ -365,13 +365,17 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
     const int ncell = w.gx * w.gy * w.gz;
     std::vector<std::vector<int>> cells(ncell);
     MeshView tmp;
+    // Cells are found as gather_tris finds them (a multiplication by the
+    // inverse cell size), so that a triangle is listed in exactly the cells a
+    // query meeting its AABB visits, whatever the cell size.
+    const float cell_inv = 1.0f / w.grid_cell;
     for (int t = 0; t < ntri; t++) {
         const float* p = &w.tri[(size_t)t * 9];
         V3 a = V3{p[0], p[1], p[2]}, b = V3{p[3], p[4], p[5]}, c = V3{p[6], p[7], p[8]};
         V3 mn = vmin(vmin(a, b), c), mx = vmax(vmax(a, b), c);
-        int x0 = (int)std::floor((mn.x - w.grid_min.x) / w.grid_cell), x1 = (int)std::floor((mx.x - w.grid_min.x) / w.grid_cell);
-        int y0 = (int)std::floor((mn.y - w.grid_min.y) / w.grid_cell), y1 = (int)std::floor((mx.y - w.grid_min.y) / w.grid_cell);
-        int z0 = (int)std::floor((mn.z - w.grid_min.z) / w.grid_cell), z1 = (int)std::floor((mx.z - w.grid_min.z) / w.grid_cell);
+        int x0 = (int)std::floor((mn.x - w.grid_min.x) * cell_inv), x1 = (int)std::floor((mx.x - w.grid_min.x) * cell_inv);
+        int y0 = (int)std::floor((mn.y - w.grid_min.y) * cell_inv), y1 = (int)std::floor((mx.y - w.grid_min.y) * cell_inv);
+        int z0 = (int)std::floor((mn.z - w.grid_min.z) * cell_inv), z1 = (int)std::floor((mx.z - w.grid_min.z) * cell_inv);
         for (int ix = std::max(x0, 0); ix <= std::min(x1, w.gx - 1); ix++)
             for (int iy = std::max(y0, 0); iy <= std::min(y1, w.gy - 1); iy++)
                 for (int iz = std::max(z0, 0); iz <= std::min(z1, w.gz - 1); iz++) cells[(ix * w.gy + iy) * w.gz + iz].push_back(t);
