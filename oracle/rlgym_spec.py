@@ -107,7 +107,10 @@ def ball_scored(ball):
 
 
 def touched_in_step(c, tick, tick_skip):
-    return bool(c.ball_hit_valid) and c.ball_hit_tick >= tick - tick_skip
+    """Player::UpdateFromCar (Player.cpp:18-23): a hit counts for the step when
+    tickCountWhenHit >= tickCount - tickSkip, in uint64 arithmetic (the
+    subtraction wraps when tickCount < tickSkip)."""
+    return bool(c.ball_hit_valid) and c.ball_hit_tick >= (tick - tick_skip) % (1 << 64)
 
 
 def _normalized(v):
@@ -137,17 +140,25 @@ def stub_rewards(cars, ball, goal, touched, w):
     return np.array(out, dtype=F)
 
 
-def step_terminal(goal, any_touch, step_dt, no_touch, timeout, goal_terminal=True):
-    """Returns (terminal type, new no-touch timer): 0 none, 1 normal, 2 truncated."""
+def step_dt(tick_skip):
+    """GameState::deltaTime = tickSkip * (1 / 120.f) (GameState.cpp:64), float32."""
+    return F(F(tick_skip) * (F(1) / F(120)))
+
+
+def step_terminal(goal, any_touch, tick_skip, no_touch, timeout, goal_terminal=True):
+    """Returns (terminal type, new no-touch timer): 0 none, 1 normal, 2
+    truncated.  NoTouchCondition::IsTerminal (NoTouchCondition.h:18-28): the
+    float32 timer is cleared by any touch in the step, otherwise advanced by
+    the step's deltaTime and compared with maxTime (>=)."""
     t = 0
     if goal_terminal and goal:
         t = 1
     if timeout > 0:
         if any_touch:
-            no_touch = 0.0
+            no_touch = F(0)
         else:
-            no_touch = float(F(no_touch) + F(step_dt))
-            if no_touch >= timeout and t == 0:
+            no_touch = F(F(no_touch) + step_dt(tick_skip))
+            if no_touch >= F(timeout) and t == 0:
                 t = 2
     return t, no_touch
 

@@ -208,7 +208,11 @@ def phys_errs(av, bv, prefix, fails, rot=True):
 
 
 def compare_tick(ref, eng, t):
-    """Returns a dict of failures for tick t (empty = parity)."""
+    """Returns a dict of failures for tick t (empty = parity).  Positions,
+    velocities and angular velocities per vector within 1e-3 + 1e-4 *
+    |reference|, rotation entries within 1e-4, continuous car scalars within
+    the same relative bar; flags, wheel contacts, event timers and ids
+    (CAR_EXACT_FIELDS), pad state and the goal flag bit for bit."""
     fails = {}
     for i, (rc, ec) in enumerate(zip(ref.cars[t], eng.cars[t])):
         phys_errs(R.car_vec(ec), R.car_vec(rc), f"car{i}", fails)
@@ -218,12 +222,17 @@ def compare_tick(ref, eng, t):
         for f in R.CAR_FLOAT_FIELDS:
             if not close(getattr(ec, f), getattr(rc, f)):
                 fails[f"car{i}.{f}"] = (getattr(rc, f), getattr(ec, f))
+        for f in R.CAR_EXACT_FIELDS:
+            if getattr(rc, f) != getattr(ec, f):
+                fails[f"car{i}.{f}"] = (getattr(rc, f), getattr(ec, f))
         if list(rc.wheels_with_contact) != list(ec.wheels_with_contact):
             fails[f"car{i}.wheels"] = (list(rc.wheels_with_contact), list(ec.wheels_with_contact))
     phys_errs(R.ball_vec(eng.balls[t]), R.ball_vec(ref.balls[t]), "ball", fails)
     for p in range(34):
         a, b = ref.pads[t][p], eng.pads[t][p]
-        if a.is_active != b.is_active or a.prev_locked_car_id != b.prev_locked_car_id or not close(b.cooldown, a.cooldown):
+        # pad state is discrete: activity, the locking car and the cooldown
+        # (BoostPad.cpp:51-105) agree bit for bit
+        if a.is_active != b.is_active or a.prev_locked_car_id != b.prev_locked_car_id or a.cooldown != b.cooldown:
             fails[f"pad{p}"] = (a.is_active, b.is_active, a.cooldown, b.cooldown)
     if ref.scored[t] != eng.scored[t]:
         fails["scored"] = (ref.scored[t], eng.scored[t])

@@ -35,8 +35,13 @@ PADS = engine.PadStates
 
 
 CAR_FLOAT_FIELDS = ["boost", "time_spent_boosting", "jump_time", "flip_time", "air_time", "air_time_since_jump",
-                    "supersonic_time", "handbrake_val", "auto_flip_timer", "auto_flip_torque_scale",
-                    "demo_respawn_timer", "car_contact_cooldown"]
+                    "supersonic_time", "handbrake_val", "auto_flip_timer", "auto_flip_torque_scale"]
+# Timers and ids of discrete events, compared bit for bit: the demolition
+# respawn countdown and the bump cooldown (Car.cpp:43-79, 133-193), the ticks
+# of the last ball hit and of its extra impulse (Arena.cpp:283-334, the source
+# of the touch flag) and the id of the last car bumped.
+CAR_EXACT_FIELDS = ["demo_respawn_timer", "car_contact_cooldown", "ball_hit_tick", "ball_hit_extra_impulse_tick",
+                    "car_contact_other_id"]
 CAR_FLAG_FIELDS = ["is_on_ground", "has_jumped", "has_double_jumped", "has_flipped", "is_flipping", "is_jumping",
                    "is_supersonic", "is_auto_flipping", "is_demoed", "world_contact", "ball_hit_valid"]
 

@@ -90,9 +90,9 @@ def test_env_outputs_match_spec(seed, oracle, cpucheck):
                 want_rew = S.stub_rewards(cars_, ball_, goal, touched, W)
                 np.testing.assert_allclose(np.array(rew), want_rew, atol=tol * 2, err_msg=f"rewards, {states}, step {step}")
                 if states == "engine":
-                    t, nt_spec_e = S.step_terminal(goal, any(touched), 8 / 120, nt_spec_e, 1.0)
+                    t, nt_spec_e = S.step_terminal(goal, any(touched), 8, nt_spec_e, 1.0)
                 else:
-                    t, nt_spec_o = S.step_terminal(goal, any(touched), 8 / 120, nt_spec_o, 1.0)
+                    t, nt_spec_o = S.step_terminal(goal, any(touched), 8, nt_spec_o, 1.0)
                 assert term == t, (states, step)
             seen.add(("touch", any(S.touched_in_step(c, tick, 8) for c in ecars)))
             seen.add(("term", term))
@@ -101,3 +101,65 @@ def test_env_outputs_match_spec(seed, oracle, cpucheck):
     finally:
         lib.cc_arena_free(h)
         oracle.ref_arena_free(o)
+
+
+def _env_arena(lib, world, cars, ball, tick):
+    h = C.c_void_p(lib.cc_arena_new(world, 1, 1))
+    lib.cc_set_tick(h, tick)
+    for i, s in enumerate(cars):
+        lib.cc_set_car(h, i, C.byref(s))
+    lib.cc_set_ball(h, C.byref(ball))
+    pads = R.PADS()
+    for i in range(34):
+        pads[i].is_active = 1
+    lib.cc_set_pads(h, pads)
+    return h
+
+
+def test_engine_touch_and_flip_boundaries(cpucheck):
+    """The engine's own touch rule, no-touch timer and HasFlipOrJump
+    (rsx_env.h, host build) at the boundaries the spec pins: a hit at
+    tick - tick_skip counts and one a tick earlier does not (Player.cpp:18-23);
+    a touch clears the no-touch timer; airTimeSinceJump < 1.25 s is strict
+    (Car.cpp:285-289)."""
+    lib, world = cpucheck
+    tick = 5000
+    ball = P.ball((0, 0, 93.15))
+    obs = (C.c_float * (2 * 89))()
+    rew = (C.c_float * 2)()
+    prev = (C.c_float * 16)()
+    for hit, want in ((tick - 8, True), (tick - 9, False), (tick - 1, True)):
+        blue = P.default_car(pos=(-3000, -4000, 17))
+        blue.ball_hit_valid, blue.ball_hit_tick = 1, hit
+        orange = P.default_car(pos=(3000, 4000, 17), team=1)
+        h = _env_arena(lib, world, [blue, orange], ball, tick)
+        nt = C.c_float(0.5)
+        term = lib.cc_env(h, prev, 8, 1.0, C.byref(nt), obs, rew)
+        lib.cc_arena_free(h)
+        assert term == 0
+        assert (nt.value == 0.0) == want, (hit, nt.value)
+        spec_t = S.touched_in_step(blue, tick, 8)
+        assert spec_t == want
+        _, spec_nt = S.step_terminal(False, spec_t, 8, np.float32(0.5), 1.0)
+        assert np.float32(nt.value) == spec_nt
+        assert abs(rew[0] - (1.0 if want else 0.0) - S.stub_rewards(
+            [blue, orange], ball, False, [False, False], W)[0]) < 1e-6
+    # no-touch truncation after exactly 15 steps of 8 ticks at a 1 s limit
+    blue = P.default_car(pos=(-3000, -4000, 17))
+    orange = P.default_car(pos=(3000, 4000, 17), team=1)
+    h = _env_arena(lib, world, [blue, orange], ball, tick)
+    nt = C.c_float(0.0)
+    terms = [lib.cc_env(h, prev, 8, 1.0, C.byref(nt), obs, rew) for _ in range(15)]
+    lib.cc_arena_free(h)
+    assert terms == [0] * 14 + [2] and nt.value == 1.0
+    # HasFlipOrJump in the self block (obs index 51 + 17)
+    for t_air, want in ((1.25, 0.0), (float(np.nextafter(np.float32(1.25), np.float32(0))), 1.0)):
+        blue = P.default_car(pos=(-3000, -4000, 600), on_ground=False)
+        blue.air_time_since_jump = t_air
+        blue.has_jumped = 1
+        h = _env_arena(lib, world, [blue, orange], ball, tick)
+        nt = C.c_float(0.0)
+        lib.cc_env(h, prev, 8, 0.0, C.byref(nt), obs, rew)
+        lib.cc_arena_free(h)
+        assert obs[51 + 17] == want, t_air
+        assert S.has_flip_or_jump(blue) == bool(want)

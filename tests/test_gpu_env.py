@@ -111,7 +111,7 @@ def test_env_step_matches_oracle_and_spec(oracle):
                     np.testing.assert_allclose(rew[2 * a:2 * a + 2], S.stub_rewards(cars_, ball_, goal, touched, W),
                                                atol=2 * tol, err_msg=f"arena {a} step {step} rewards vs {name}")
                     if name == "oracle":
-                        t, nt[a] = S.step_terminal(goal, any(touched), 8 / 120, nt[a], 1.0)
+                        t, nt[a] = S.step_terminal(goal, any(touched), 8, nt[a], 1.0)
                         assert term[a] == t, (a, step)
                         seen_term.add(int(t))
         assert {1, 2} & seen_term, seen_term

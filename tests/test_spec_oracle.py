@@ -111,16 +111,59 @@ def test_rewards_closed_form():
 
 
 def test_terminals():
-    t, nt = S.step_terminal(False, False, 8 / 120, 0.0, 0.0)
+    t, nt = S.step_terminal(False, False, 8, 0.0, 0.0)
     assert t == 0 and nt == 0.0
     nt, steps = 0.0, 0
     while True:
         steps += 1
-        t, nt = S.step_terminal(False, False, 8 / 120, nt, 1.0)
+        t, nt = S.step_terminal(False, False, 8, nt, 1.0)
         if t:
             break
-    assert t == 2 and steps in (15, 16)
-    t, nt = S.step_terminal(True, False, 8 / 120, 0.99, 1.0)
+    # float32: 8 * (1 / 120.f) = 0.06666667, and 15 of them sum to exactly 1.0f
+    assert S.step_dt(8) == np.float32(0.06666667)
+    assert t == 2 and steps == 15 and nt == np.float32(1.0)
+    t, nt = S.step_terminal(True, False, 8, 0.99, 1.0)
     assert t == 1
-    t, nt = S.step_terminal(False, True, 8 / 120, 0.99, 1.0)
+    t, nt = S.step_terminal(False, True, 8, 0.99, 1.0)
     assert t == 0 and nt == 0.0
+    # a goal takes priority over the truncation (EnvSet.cpp:178-200)
+    t, _ = S.step_terminal(True, False, 8, 0.99, 1.0)
+    assert t == 1
+
+
+def _hit_car(tick_hit, valid=1):
+    c = P.default_car()
+    c.ball_hit_valid = valid
+    c.ball_hit_tick = tick_hit
+    return c
+
+
+def test_touched_in_step_boundary():
+    """Player.cpp:18-23: tickCountWhenHit >= tickCount - tickSkip.  A hit at
+    tick - tick_skip counts, one tick earlier does not; an invalid hit never
+    counts; with tickCount < tickSkip the uint64 subtraction wraps and only a
+    hit tick at or past the wrapped value counts."""
+    assert S.touched_in_step(_hit_car(1000 - 8), 1000, 8)
+    assert not S.touched_in_step(_hit_car(1000 - 8 - 1), 1000, 8)
+    assert S.touched_in_step(_hit_car(999), 1000, 8)
+    assert not S.touched_in_step(_hit_car(1000 - 8, valid=0), 1000, 8)
+    assert not S.touched_in_step(_hit_car(2), 3, 8)
+    assert S.touched_in_step(_hit_car(2**64 - 5), 3, 8)
+    assert not S.touched_in_step(_hit_car(2**64 - 6), 3, 8)
+
+
+def test_has_flip_or_jump_boundary():
+    """Car.cpp:285-289: on the ground, or no flip / double jump yet and
+    airTimeSinceJump < 1.25 s (strict)."""
+    c = P.default_car(on_ground=False)
+    c.air_time_since_jump = 1.25
+    assert not S.has_flip_or_jump(c)
+    c.air_time_since_jump = float(np.nextafter(np.float32(1.25), np.float32(0)))
+    assert S.has_flip_or_jump(c)
+    c.has_flipped = 1
+    assert not S.has_flip_or_jump(c)
+    c.has_flipped, c.has_double_jumped = 0, 1
+    assert not S.has_flip_or_jump(c)
+    c.is_on_ground = 1
+    c.air_time_since_jump = 5.0
+    assert S.has_flip_or_jump(c)
