@@ -139,26 +139,40 @@ RSX_HD void manifold_refresh(Manifold& m, const Xf& trA, const Xf& trB) {
 }
 
 // --- manifold list of the tick ------------------------------------------------
-constexpr int kMaxManifolds = 40;
-constexpr int kMaxLive = 12;
-
+// Every pair the dispatch opens gets an entry, in dispatch order (empty ones
+// too: they take part in the island sort); only pairs that hold contacts keep
+// a manifold's storage.  kCap is the most pairs a tick can open: each body
+// against every mesh and plane, and every pair of dynamic bodies.  kLive, the
+// manifolds with contacts a tick can hold, is far below the theoretical bound
+// (the contact-heavy 1v1 workload peaks at 8).  Once kLive manifolds hold
+// contacts, a further pair gets no storage and is skipped; opening one sets
+// `overflow` (conservatively: the pair may have had no contact), which the
+// engine reports per arena.
+template <int MAXC>
 struct MfList {
+    static constexpr int kCap = (MAXC + 1) * (kNumMeshes + kNumPlanes) + (MAXC + 1) * MAXC / 2;
+    static constexpr int kLive = 12 + 2 * (MAXC - 2);
     int count;
-    int8_t body0[kMaxManifolds], body1[kMaxManifolds];  // 0 ball, 1+i car i, -1 static
-    int8_t slot[kMaxManifolds];                          // index into live, -1 empty
+    int8_t body0[kCap], body1[kCap];  // 0 ball, 1+i car i, -1 static
+    int8_t slot[kCap];                // index into live, -1 empty
     int nlive;
-    Manifold live[kMaxLive];
+    bool overflow;                    // some pair found no storage since the flag was cleared
+    Manifold live[kLive];
 };
 
 // Returns the pair's index, or -1 when the pair list is full (the pair is
-// then skipped for this tick).
-RSX_HD int mf_open(MfList& L, int b0, int b1, float breaking) {
-    if (L.count >= kMaxManifolds) return -1;
+// then skipped for this tick and the overflow flag set).
+template <int MAXC>
+RSX_HD int mf_open(MfList<MAXC>& L, int b0, int b1, float breaking) {
+    if (L.count >= MfList<MAXC>::kCap) {
+        L.overflow = true;
+        return -1;
+    }
     int k = L.count++;
     L.body0[k] = (int8_t)b0;
     L.body1[k] = (int8_t)b1;
     L.slot[k] = -1;
-    if (L.nlive < kMaxLive) {
+    if (L.nlive < MfList<MAXC>::kLive) {
         int s = L.nlive;
         Manifold& m = L.live[s];
         m.body0 = b0;
@@ -166,11 +180,14 @@ RSX_HD int mf_open(MfList& L, int b0, int b1, float breaking) {
         m.n = 0;
         m.breaking = breaking;
         L.slot[k] = (int8_t)s;
+    } else {
+        L.overflow = true;
     }
     return k;
 }
 // Keep the manifold's storage only if it ended the tick with contacts.
-RSX_HD void mf_close(MfList& L, int k) {
+template <int MAXC>
+RSX_HD void mf_close(MfList<MAXC>& L, int k) {
     if (k < 0) return;
     int s = L.slot[k];
     if (s < 0) return;

@@ -306,6 +306,7 @@ void arena_info_from(const Col& c, int maxc, int team, rsx_arena_info& out) {
     out.episode = c.get(o + kArEpisode);
     uint32_t f = c.get(o + kArFlags);
     out.goal_scored = (f & kFlagGoal) ? 1 : 0;
+    out.contact_overflow = (f & kFlagContactOverflow) ? 1 : 0;
     out.goal_team = (f & kFlagGoal) ? ((f & kFlagGoalOrange) ? 1 : 0) : -1;
 }
 

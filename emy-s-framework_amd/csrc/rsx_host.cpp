@@ -167,13 +167,20 @@ static void pad_geometry(PadGeom* pads) {
 
 static void internal_edge_info(HostWorld& w);
 
-// Leaf order of the quantized BVH that btBvhTriangleMeshShape builds over one
-// mesh (btOptimizedBvh::build, btQuantizedBvh::setQuantizationValues /
+// Traversal order of the quantized BVH that btBvhTriangleMeshShape builds over
+// one mesh (btOptimizedBvh::build, btQuantizedBvh::setQuantizationValues /
 // quantize / unQuantize / calcSplittingAxis / sortAndCalcSplittingIndex /
-// buildTree, bullet3-3.24 BroadphaseCollision/btQuantizedBvh.cpp).  The
-// stackless traversal visits overlapping leaves in node-array order, which is
-// the final order of the leaf array after the in-place partitions; the
-// returned rank[t] is triangle t's position in it.
+// buildTree / updateSubtreeHeaders, bullet3-3.24
+// BroadphaseCollision/btQuantizedBvh.cpp).  RocketSim's Bullet always walks
+// the tree subtree header by subtree header
+// (reportAabbOverlappingNodex -> walkStacklessQuantizedTreeCacheFriendly,
+// btQuantizedBvh.cpp:307-325), and node-array order inside each header's
+// subtree; node-array order is the final order of the leaf array after the
+// in-place partitions.  A header is recorded for each child of at most 128
+// nodes (2048 bytes) of a node of more than 128 nodes, when that node's
+// build finishes -- so a small left child is visited after the headers
+// inside its large right sibling -- or for the whole tree when no node is
+// that large.  The returned rank[t] is triangle t's position in that order.
 namespace {
 struct QBvh {
     V3 amin, amax, quant;
@@ -248,12 +255,20 @@ struct QBvh {
         if (unbalanced) split = s + (n >> 1);
         return split;
     }
+    // Subtree headers as leaf ranges [first, second), in the order Bullet
+    // appends them.  A subtree over k leaves has 2k - 1 nodes.
+    static constexpr int kMaxSubtreeNodes = 2048 / 16;  // MAX_SUBTREE_SIZE_IN_BYTES / sizeof(btQuantizedBvhNode)
+    std::vector<std::pair<int, int>> headers;
     void build(int s, int e) {
         if (e - s == 1) return;
         int axis = split_axis(s, e);
         int split = split_index(s, e, axis);
         build(s, split);
         build(split, e);
+        if (2 * (e - s) - 1 > kMaxSubtreeNodes) {
+            if (2 * (split - s) - 1 <= kMaxSubtreeNodes) headers.push_back({s, split});
+            if (2 * (e - split) - 1 <= kMaxSubtreeNodes) headers.push_back({split, e});
+        }
     }
 };
 }  // namespace
@@ -279,8 +294,11 @@ static std::vector<int> bvh_leaf_rank(const float* tri, int nt, V3 mn, V3 mx) {
         q.leaves[t].tri = t;
     }
     q.build(0, nt);
-    std::vector<int> rank(nt);
-    for (int i = 0; i < nt; i++) rank[q.leaves[i].tri] = i;
+    if (q.headers.empty()) q.headers.push_back({0, nt});
+    std::vector<int> rank(nt, -1);
+    int r = 0;
+    for (const auto& h : q.headers)
+        for (int i = h.first; i < h.second; i++) rank[q.leaves[i].tri] = r++;
     return rank;
 }
 

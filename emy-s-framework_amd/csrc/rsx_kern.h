@@ -29,9 +29,11 @@
 
 namespace rsx {
 
-// Arena goal flags word (kArFlags): a goal has been scored this episode, and
-// whether it went into the orange goal's side (ball y <= 0).
-constexpr uint32_t kFlagGoal = 1u, kFlagGoalOrange = 2u;
+// Arena flags word (kArFlags): a goal has been scored this episode, and
+// whether it went into the orange goal's side (ball y <= 0); a tick ran out of
+// contact-manifold storage (MfList::overflow) since the arena was last reset
+// by rsx_reset_arena / rsx_reset_all_arenas (kept across auto-resets).
+constexpr uint32_t kFlagGoal = 1u, kFlagGoalOrange = 2u, kFlagContactOverflow = 4u;
 
 // One team-size bucket.
 struct BucketArgs {

@@ -63,11 +63,13 @@ __device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs&
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
+    W.L.overflow = false;
     uint32_t flags = col.get(Layout<MAXC>::kArenaBase + kArFlags);
     for (int t = 0; t < ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    if (W.L.overflow) flags |= kFlagContactOverflow;
     cull_flush(T, W);
     if (!la.own) return;
     pack_arena<MAXC>(col, A);
@@ -111,11 +113,13 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
+    W.L.overflow = false;
     uint32_t flags = col.get(L::kArenaBase + kArFlags);
     for (int t = 0; t < V.ticks; t++) {
         if (arena_tick(A, T, W) && !(flags & kFlagGoal))
             flags |= kFlagGoal | (A.ball.xf.o.y > 0.f ? 0u : kFlagGoalOrange);
     }
+    if (W.L.overflow) flags |= kFlagContactOverflow;
     cull_flush(T, W);
     if (!la.own) return;
     const EnvConfig& E = V.E;
@@ -133,7 +137,7 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     if (tt != kNotTerminal && E.auto_reset) {
         arena_reset_kickoff<MAXC>(A, G.cg);
         A.tick = 0;
-        flags = 0;
+        flags &= kFlagContactOverflow;
         no_touch = 0.f;
         episode++;
         for (int i = 0; i < A.ncars; i++)

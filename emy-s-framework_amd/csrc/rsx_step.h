@@ -12,9 +12,11 @@
 namespace rsx {
 
 // ---- Bullet's unstable quicksort (btAlignedObjectArray::quickSortInternal) ----
+template <int CAP>
 RSX_HD void bt_quicksort(int* key, int* val, int lo, int hi) {
-    // explicit stack: ranges are tiny (<= kMaxManifolds)
-    int stk_lo[64], stk_hi[64];
+    // explicit stack: every pending range is a distinct, non-empty slice, so
+    // CAP (the array length) entries always suffice
+    int stk_lo[CAP], stk_hi[CAP];
     int sp = 0;
     stk_lo[sp] = lo;
     stk_hi[sp] = hi;
@@ -36,8 +38,8 @@ RSX_HD void bt_quicksort(int* key, int* val, int lo, int hi) {
         } while (i <= j);
         // recursion order of the reference: (L, j) first, then (i, H); a stack
         // reverses, so push (i, H) first.
-        if (i < H && sp < 64) { stk_lo[sp] = i; stk_hi[sp] = H; sp++; }
-        if (L < j && sp < 64) { stk_lo[sp] = L; stk_hi[sp] = j; sp++; }
+        if (i < H && sp < CAP) { stk_lo[sp] = i; stk_hi[sp] = H; sp++; }
+        if (L < j && sp < CAP) { stk_lo[sp] = L; stk_hi[sp] = j; sp++; }
     }
 }
 
@@ -52,7 +54,7 @@ RSX_HD int uf_find(int* id, int x) {
 
 template <int MAXC>
 struct TickScratch {
-    MfList L;
+    MfList<MAXC> L;
     Solver<MAXC> S;
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
     int cand[kMaxCand];
@@ -431,7 +433,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     }
 
     // ---- pairs, manifolds, contacts (dispatch order) ----
-    MfList& L = W.L;
+    MfList<MAXC>& L = W.L;
     L.count = 0;
     L.nlive = 0;
     int uf[MAXC + 1];
@@ -447,9 +449,15 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         bool demoed = (d >= 1) && !A.car[d - 1].active;
         const int cell = bp_cell_index(T.mv, W.bmin[d]);
         uint32_t mask = T.mv.bp_meshes[cell];
-        // statics: meshes then planes.  The body's candidate triangles are
-        // gathered once; a mesh with none of them cannot add a contact, and its
-        // pair would only carry an empty manifold, so it is skipped.
+        // statics: meshes then planes.  Every mesh pair that passes the AABB
+        // tests opens its manifold, in this order, even when no triangle can
+        // reach the body: Bullet hands every pair's manifold, empty or not, to
+        // the island sort (btSimulationIslandManager::buildIslands), whose
+        // unstable quicksort orders the manifolds that do hold contacts -- and
+        // so the solver's rows -- by where the empty ones sit.  The narrowphase
+        // runs only for meshes with candidate triangles (`mask`), gathered once
+        // per body; a culled body opens its manifolds and skips the rest.
+        const uint32_t open = demoed ? 0u : mask;
         int ncand = 0;
         if (!demoed && mask && active[d] && !far[d]) {
             if (d == 0) {
@@ -472,7 +480,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
 #endif
         if (demoed) continue;  // no contact response: never solved
         for (int ms = 0; ms < kNumMeshes; ms++) {
-            if (!((mask >> ms) & 1)) continue;
+            if (!((open >> ms) & 1)) continue;
             const float* ma = T.mv.mesh_aabb + ms * 6;
             const float th = kContactBreakingThreshold;
             V3 smn = V3{ma[0], ma[1], ma[2]}, smx = V3{ma[3], ma[4], ma[5]};  // shape AABB
@@ -491,8 +499,10 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
-                if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand);
-                else car_vs_mesh(A, T, W, m, d - 1, ms, ncand);
+                if ((mask >> ms) & 1) {
+                    if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand);
+                    else car_vs_mesh(A, T, W, m, d - 1, ms, ncand);
+                }
             }
             mf_close(L, k);
         }
@@ -632,7 +642,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     for (int i = 0; i < nb; i++) in_solver[i] = island_awake[tag[i]] && !((i >= 1) && !A.car[i - 1].active);
     bool ball_woken = A.ball.sleeping && in_solver[0];
 
-    int keys[kMaxManifolds], order[kMaxManifolds];
+    constexpr int kCap = MfList<MAXC>::kCap;
+    int keys[kCap], order[kCap];
     int nm = 0;
     for (int k = 0; k < L.count; k++) {
         int b0 = L.body0[k];
@@ -640,7 +651,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         order[nm] = k;
         nm++;
     }
-    if (nm > 1) bt_quicksort(keys, order, 0, nm - 1);
+    if (nm > 1) bt_quicksort<kCap>(keys, order, 0, nm - 1);
 
     // ---- solver setup ----
     Solver<MAXC>& S = W.S;
@@ -892,7 +903,15 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
 #endif
     // ball zero-velocity sleep
     A.ball.sleeping = (len2(A.ball.vel) == 0 && len2(A.ball.ang) == 0);
-    for (int i = 0; i < A.ncars; i++) {
+    // Cars are visited last-added first, as Arena::Step's loops over
+    // Arena::_cars (a std::unordered_set<Car*>) visit them: libstdc++ links a
+    // node inserted into an empty bucket at the head of the set's list, so
+    // iteration runs in reverse insertion order unless two car pointers share
+    // a bucket (the oracle's arenas: every 1v1, 199 of 200 2v2 and 3v3).  The
+    // order decides which car's wheel rays see another car's response flag
+    // as of this tick, whose impulses a car on another car reads, and which
+    // of two cars on one pad locks it.
+    for (int i = A.ncars - 1; i >= 0; i--) {
         Car& c = A.car[i];
         Controls& k = c.controls;
         k.throttle = clampf(k.throttle, -1, 1);
@@ -924,7 +943,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     M3 rot0[MAXC];
     for (int i = 0; i < A.ncars; i++) rot0[i] = A.car[i].xf.b;
     arena_world_step(A, T, W, far);
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = A.ncars - 1; i >= 0; i--) {  // post-tick and pads, last-added car first
         Car& c = A.car[i];
         // A car demolished during this step skips Car::_PostTickUpdate, which
         // is where its public rotation would have followed the body; the body

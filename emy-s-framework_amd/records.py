@@ -64,7 +64,8 @@ PadStates = BoostPadState * NUM_PADS
 
 class ArenaInfo(C.Structure):
     _fields_ = [("tick_count", C.c_uint64), ("num_cars", C.c_int32), ("team_size", C.c_int32),
-                ("goal_scored", C.c_uint8), ("_pad", C.c_uint8 * 3), ("goal_team", C.c_int32),
+                ("goal_scored", C.c_uint8), ("contact_overflow", C.c_uint8), ("_pad", C.c_uint8 * 2),
+                ("goal_team", C.c_int32),
                 ("no_touch_time", C.c_float), ("episode", C.c_uint32)]
 
 

@@ -111,7 +111,10 @@ typedef struct rsx_arena_info {
     int32_t num_cars;
     int32_t team_size;
     uint8_t goal_scored;    /* set once the ball is past a goal line (Arena::IsBallScored); cleared by reset */
-    uint8_t _pad[3];
+    uint8_t contact_overflow; /* no counterpart in the reference: set when a tick had more contact manifolds than
+                                 the engine stores (their contacts were dropped); cleared by rsx_reset_arena /
+                                 rsx_reset_all_arenas, kept across the env step's auto-reset */
+    uint8_t _pad[2];
     int32_t goal_team;      /* -1 none; 0 = blue scored (ball at +y), 1 = orange scored */
     float no_touch_time;    /* seconds since any car touched the ball */
     uint32_t episode;       /* number of resets this arena has seen */

@@ -168,11 +168,31 @@ int ref_init(const char* mesh_dir) {
     return 0;
 }
 
+// Arena::Step visits Arena::_cars, a std::unordered_set<Car*>, whose order
+// libstdc++ makes the reverse of insertion unless two car pointers share a
+// bucket, which happens for an occasional 2v2 / 3v3 arena depending on where
+// the cars were allocated.  The test arenas pin the usual order: when the set
+// does not already visit the cars last-added first, it is rebuilt with one
+// effective bucket (car pointers are 16-byte aligned and the table keeps two
+// buckets at a load factor this high), where each insertion goes to the front.
+static void pin_car_order(RefArena* h) {
+    auto& set = h->arena->_cars;
+    size_t k = h->cars.size();
+    bool reversed = true;
+    for (Car* c : set) reversed = reversed && k > 0 && c == h->cars[--k];
+    if (reversed) return;
+    set.clear();
+    set.max_load_factor(1024.f);
+    set.rehash(0);
+    for (Car* c : h->cars) set.insert(c);
+}
+
 void* ref_arena_new(int n_blue, int n_orange) {
     auto* h = new RefArena();
     h->arena = Arena::Create(GameMode::SOCCAR);
     for (int i = 0; i < n_blue; i++) h->cars.push_back(h->arena->AddCar(Team::BLUE, CAR_CONFIG_OCTANE));
     for (int i = 0; i < n_orange; i++) h->cars.push_back(h->arena->AddCar(Team::ORANGE, CAR_CONFIG_OCTANE));
+    pin_car_order(h);
     return h;
 }
 
@@ -472,4 +492,96 @@ extern "C" int64_t ref_run_random(void** arenas, int n, int steps, int tick_skip
         }
     }
     return done;
+}
+
+// Ids of the arena's cars in the order Arena::Step visits them (the
+// iteration order of Arena::_cars, a std::unordered_set<Car*>).  Returns the
+// number written.
+extern "C" int ref_car_order(void* p, uint32_t* out, int cap) {
+    auto* h = (RefArena*)p;
+    int n = 0;
+    for (Car* c : h->arena->GetCars())
+        if (n < cap) out[n++] = c->id;
+    return n;
+}
+
+// Contact tracing (a diagnostic): while on, every point Bullet adds to a
+// manifold is logged as it leaves btManifoldResult::addContactPoint, before
+// and after RocketSim's contact-added callback (which corrects normals against
+// internal mesh edges): 16 floats per point, body codes of objects A and B,
+// the triangle indices (index0, index1), position on B (3), normal before (3)
+// and after (3) the callback, depth, and the point's slot in its manifold
+// (-1 when not found).  Re-arm after creating an arena: Arena's constructor
+// installs its own callback.
+namespace {
+std::vector<float> g_trace;
+RefArena* g_trace_arena = nullptr;
+bool trace_cb(btManifoldPoint& cp, const btCollisionObjectWrapper* a, int pa, int ia, const btCollisionObjectWrapper* b,
+              int pb, int ib) {
+    btVector3 n0 = cp.m_normalWorldOnB;
+    bool r = Arena::_BulletContactAddedCallback(cp, a, pa, ia, b, pb, ib);
+    RefArena* h = g_trace_arena;
+    float slot = -1;
+    auto* disp = h->arena->_bulletWorld.getDispatcher();
+    for (int m = 0; m < disp->getNumManifolds(); m++) {
+        btPersistentManifold* mf = disp->getManifoldByIndexInternal(m);
+        for (int i = 0; i < mf->getNumContacts(); i++)
+            if (&mf->getContactPoint(i) == &cp) slot = (float)i;
+    }
+    const float v[16] = {(float)body_code(h, a->getCollisionObject()), (float)body_code(h, b->getCollisionObject()),
+                         (float)ia, (float)ib, cp.m_positionWorldOnB.x(), cp.m_positionWorldOnB.y(),
+                         cp.m_positionWorldOnB.z(), n0.x(), n0.y(), n0.z(), cp.m_normalWorldOnB.x(),
+                         cp.m_normalWorldOnB.y(), cp.m_normalWorldOnB.z(), cp.m_distance1, slot, 0.f};
+    g_trace.insert(g_trace.end(), v, v + 16);
+    return r;
+}
+}  // namespace
+
+extern "C" void ref_trace_contacts(void* p, int on) {
+    g_trace.clear();
+    g_trace_arena = (RefArena*)p;
+    gContactAddedCallback = on ? &trace_cb : &Arena::_BulletContactAddedCallback;
+}
+
+extern "C" int ref_trace_read(float* out, int cap) {
+    int n = (int)g_trace.size() < cap ? (int)g_trace.size() : cap;
+    memcpy(out, g_trace.data(), n * sizeof(float));
+    return n;
+}
+
+// Triangle indices of soccar mesh `mesh` in the order of its quantized BVH's
+// leaf nodes (btQuantizedBvh node array, which the stackless traversal walks
+// in order).  Returns the number of leaves.
+extern "C" int ref_mesh_leaf_order(int mesh, int32_t* out, int cap) {
+    auto& shapes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (mesh < 0 || mesh >= (int)shapes.size()) return -1;
+    btOptimizedBvh* bvh = shapes[mesh]->getOptimizedBvh();
+    if (!bvh || !bvh->isQuantized()) return -2;
+    const QuantizedNodeArray& nodes = bvh->getQuantizedNodeArray();
+    int n = 0;
+    for (int i = 0; i < nodes.size(); i++)
+        if (nodes[i].isLeafNode()) {
+            if (n < cap) out[n] = nodes[i].getTriangleIndex();
+            n++;
+        }
+    return n;
+}
+
+// Triangle indices of soccar mesh `mesh` that btBvhTriangleMeshShape::
+// processAllTriangles reports for the AABB (Bullet units), in report order.
+extern "C" int ref_mesh_query(int mesh, const float* aabb6, int32_t* out, int cap) {
+    struct Cb : btTriangleCallback {
+        int32_t* out; int cap; int n = 0;
+        void processTriangle(btVector3*, int, int idx) override {
+            if (n < cap) out[n] = idx;
+            n++;
+        }
+    } cb;
+    cb.out = out;
+    cb.cap = cap;
+    auto& shapes = RocketSim::GetArenaCollisionShapes(GameMode::SOCCAR);
+    if (mesh < 0 || mesh >= (int)shapes.size()) return -1;
+    shapes[mesh]->processAllTriangles(&cb, btVector3(aabb6[0], aabb6[1], aabb6[2]),
+                                      btVector3(aabb6[3], aabb6[4], aabb6[5]));
+    return cb.n;
 }

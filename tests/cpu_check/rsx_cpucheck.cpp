@@ -182,7 +182,7 @@ void cc_soa_roundtrip(void* p, int fill) {
 // oracle's ref_dump_contacts (body codes: static s -> s, ball 100, car i 200+i).
 int cc_dump_contacts(void* p, float* out, int max_floats) {
     auto* a = (CcArena*)p;
-    const MfList& L = a->W.L;
+    const MfList<6>& L = a->W.L;
     int n = 0;
     auto code = [](int b) { return b >= kBodyStatic ? b - kBodyStatic : (b == 0 ? 100 : 200 + b - 1); };
     for (int k = 0; k < L.count; k++) {
@@ -304,3 +304,16 @@ extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, 
     for (int i = 0; i < A.ncars; i++) obs_row<6>(A, i, pa, E, obs + i * row);
     return t;
 }
+
+// Rank of every triangle within its mesh's BVH traversal order (the engine's
+// reconstruction of btQuantizedBvh's leaf order); returns the triangle count.
+extern "C" int cc_tri_rank(void* w, int32_t* out, int cap) {
+    const HostWorld& h = ((CcWorld*)w)->w;
+    int n = (int)h.tri_rank.size();
+    for (int t = 0; t < n && t < cap; t++) out[t] = h.tri_rank[t];
+    return n;
+}
+
+// Whether any tick since the arena was created ran out of contact-manifold
+// storage (MfList::overflow).
+extern "C" int cc_contact_overflow(void* p) { return ((CcArena*)p)->W.L.overflow ? 1 : 0; }

@@ -101,6 +101,10 @@ def load_oracle():
         ("ref_num_cars", C.c_int, [VP]),
         ("ref_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
         ("ref_mesh_tri_info", C.c_int, [C.c_int, C.POINTER(C.c_float), C.c_int]),
+        ("ref_car_order", C.c_int, [VP, C.POINTER(C.c_uint32), C.c_int]),
+        ("ref_mesh_query", C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int]),
+        ("ref_trace_contacts", None, [VP, C.c_int]),
+        ("ref_trace_read", C.c_int, [C.POINTER(C.c_float), C.c_int]),
     ])
     if lib is not None:
         assert lib.ref_init(MESH_DIR.encode()) == 0
@@ -130,6 +134,8 @@ def load_cpucheck():
         ("cc_dump_contacts", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
         ("cc_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
         ("cc_tri_info", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
+        ("cc_contact_overflow", C.c_int, [VP]),
+        ("cc_tri_rank", C.c_int, [VP, C.POINTER(C.c_int32), C.c_int]),
         ("cc_point_tri_dist2", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
         ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
