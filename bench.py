@@ -52,66 +52,156 @@ def _import_package():
     return mod
 
 
-def cpu_baseline(seconds, threads, tick_skip=8):
-    """The reference CPU RocketSim (oracle/_ref/librsx_ref.so) on this host:
-    64 1v1 arenas from kickoffs, random default-table actions, Step(8) per env
-    step, arenas split over `threads` threads; runs for about `seconds`."""
+class _HostRecords:
+    """Whole-batch record arrays on the host (1v1 arenas), for the workload
+    generators, which read and write every arena's records at once."""
+
+    def __init__(self, eng, n):
+        import numpy as np
+        self.num_arenas, self.team_size, self.num_cars = n, [1] * n, 2 * n
+        self.balls = (eng.BallState * n)()
+        self.cars = (eng.CarState * (2 * n))()
+        v = np.ctypeslib.as_array(self.cars)
+        v["rot"]["forward"]["x"] = v["rot"]["right"]["y"] = v["rot"]["up"]["z"] = 1
+        v["pos"]["z"], v["boost"], v["is_on_ground"] = 17, 33.3, 1
+        v["team"] = np.arange(2 * n) & 1
+        v["ball_hit_tick"] = v["ball_hit_extra_impulse_tick"] = 2**64 - 1
+        self.pads = (eng.BoostPadState * (34 * n))()
+
+    def GetAllBallStates(self): return self.balls
+    def SetAllBallStates(self, s): self.balls = s
+    def GetAllCarStates(self): return self.cars
+    def SetAllCarStates(self, s): self.cars = s
+    def GetAllBoostPadStates(self): return self.pads
+    def SetAllBoostPadStates(self, s): self.pads = s
+
+
+def host_info():
+    """Cores the CPU baseline may use and what they are."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
+    return {"cpu_model": model, "nproc": nproc, "affinity": avail, "omp_num_threads": share or None,
+            "threads": max(1, threads)}
+
+
+def cpu_baseline(seconds, threads, workload, tick_skip=8, sample_every=32):
+    """The reference CPU RocketSim (oracle/_ref/librsx_ref.so) on this host,
+    running the same env step the engine fuses: a uniform default-table action
+    per car, Arena::Step(tick_skip), DefaultObs rows, the four stub rewards,
+    goal / no-touch (10 s) terminals and the kickoff reset of terminal arenas
+    (ref_env_run).  Arenas: every `sample_every`-th arena of the benchmark's
+    own 65536-arena workload (per-arena cost does not depend on the batch
+    size), split over `threads` threads; runs for about `seconds`.  Also times
+    the C1 shape (64 1v1 arenas from kickoffs) for a few seconds."""
     path = os.path.join(ROOT, "oracle", "_ref", "librsx_ref.so")
     if not os.path.exists(path):
         return None
     eng = _import_package()
+    from emy_s_framework_amd.workloads import random_field_states, stress_states
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import rlgym_spec
     lib = C.CDLL(path)
+    VP = C.c_void_p
     lib.ref_init.argtypes = [C.c_char_p]
-    lib.ref_arena_new.restype = C.c_void_p
+    lib.ref_arena_new.restype = VP
     lib.ref_arena_new.argtypes = [C.c_int, C.c_int]
-    lib.ref_arena_free.argtypes = [C.c_void_p]
-    lib.ref_reset_kickoff.argtypes = [C.c_void_p, C.c_int]
-    lib.ref_run_random.restype = C.c_int64
-    lib.ref_run_random.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_uint32,
-                                   C.POINTER(eng.CarControls), C.c_int]
+    lib.ref_arena_free.argtypes = [VP]
+    lib.ref_reset_kickoff.argtypes = [VP, C.c_int]
+    lib.ref_step.argtypes = [VP, C.c_int]
+    lib.ref_set_car.argtypes = [VP, C.c_int, C.POINTER(eng.CarState)]
+    lib.ref_set_ball.argtypes = [VP, C.POINTER(eng.BallState)]
+    lib.ref_set_pads.argtypes = [VP, C.POINTER(eng.BoostPadState)]
+    lib.ref_set_tick.argtypes = [VP, C.c_uint64]
+    lib.ref_env_run.restype = C.c_int64
+    lib.ref_env_run.argtypes = [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(eng.CarControls),
+                                C.c_int, C.POINTER(C.c_int32), C.c_float]
     if lib.ref_init(eng.MESH_DIR.encode()) != 0:
         raise RuntimeError("oracle mesh load failed")
     lut = eng.default_action_lut()
-    n_arenas = 64
-    arenas = [lib.ref_arena_new(1, 1) for _ in range(n_arenas)]
-    for i, h in enumerate(arenas):
-        lib.ref_reset_kickoff(h, i)
-    per = [arenas[t::threads] for t in range(threads)]
-    arrs = [(C.c_void_p * len(p))(*p) for p in per]
-    steps = [1]
-    done = [0] * threads
-    per_thread_s = [0.0] * threads
-    ready = threading.Barrier(threads + 1)
-    go = threading.Barrier(threads + 1)
+    pad_map = (C.c_int32 * 34)(*[int(x) for x in rlgym_spec.PAD_MAP])
 
-    def work(t):
-        # RocketSim builds per-thread caches on a thread's first step: do that
-        # (and time one step) before the timed region.
-        lib.ref_run_random(arrs[t], len(per[t]), 1, tick_skip, 7 + t, lut, len(lut))
-        t0 = time.perf_counter()
-        lib.ref_run_random(arrs[t], len(per[t]), 2, tick_skip, 77 + t, lut, len(lut))
-        per_thread_s[t] = (time.perf_counter() - t0) / 2
+    def timed(n_arenas, make, secs, seed0):
+        """Env steps per second of `threads` threads over n_arenas arenas:
+        thread t builds arenas t, t + threads, ... with make(k) (Bullet world
+        set-up is the slow part), runs one untimed step (per-thread caches),
+        then chunks of 2 env steps until `secs` have passed; frees its arenas."""
+        per = [list(range(t, n_arenas, threads)) for t in range(threads)]
+        per = [p for p in per if p]
+        nt = len(per)
+        done = [0] * nt
+        chunks = [0] * nt
+        ready, go = threading.Barrier(nt + 1), threading.Barrier(nt + 1)
+        deadline = [0.0]
+
+        def work(t):
+            hs = [make(k) for k in per[t]]
+            arr = (VP * len(hs))(*hs)
+            lib.ref_env_run(arr, len(hs), 1, tick_skip, seed0 + t, lut, len(lut), pad_map, 10.0)
+            ready.wait()
+            go.wait()
+            while time.perf_counter() < deadline[0]:
+                done[t] += lib.ref_env_run(arr, len(hs), 2, tick_skip, seed0 + 100 + 7919 * chunks[t] + t, lut,
+                                           len(lut), pad_map, 10.0)
+                chunks[t] += 1
+            for h in hs:
+                lib.ref_arena_free(h)
+
+        ths = [threading.Thread(target=work, args=(t,)) for t in range(nt)]
+        for th in ths:
+            th.start()
         ready.wait()
+        t0 = time.perf_counter()
+        deadline[0] = t0 + secs
         go.wait()
-        done[t] = lib.ref_run_random(arrs[t], len(per[t]), steps[0], tick_skip, 1000 + t, lut, len(lut))
+        for th in ths:
+            th.join()
+        return sum(done), max(time.perf_counter() - t0, 1e-9), min(chunks) * 2
 
-    ths = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
-    for th in ths:
-        th.start()
-    ready.wait()
-    steps[0] = max(2, int(seconds / max(max(per_thread_s), 1e-6)))
-    go.wait()
-    t0 = time.perf_counter()
-    for th in ths:
-        th.join()
-    dt = time.perf_counter() - t0
-    steps = steps[0]
-    for h in arenas:
-        lib.ref_arena_free(h)
-    total = sum(done)
+    # the benchmark workload, sampled
+    full = 65536
+    recs = _HostRecords(eng, full)
+    (stress_states if workload == "stress" else random_field_states)(recs, 4)
+    idx = list(range(0, full, sample_every))
+    psz = C.sizeof(eng.BoostPadState)
+
+    def make_workload(k):
+        a = idx[k]
+        h = lib.ref_arena_new(1, 1)
+        lib.ref_step(h, 1)
+        lib.ref_set_tick(h, 0)
+        for i in range(2):
+            lib.ref_set_car(h, i, C.byref(recs.cars[2 * a + i]))
+        lib.ref_set_ball(h, C.byref(recs.balls[a]))
+        lib.ref_set_pads(h, C.cast(C.addressof(recs.pads) + a * 34 * psz, C.POINTER(eng.BoostPadState)))
+        return h
+
+    def make_kickoff(k):
+        h = lib.ref_arena_new(1, 1)
+        lib.ref_reset_kickoff(h, k)
+        return h
+
+    total, dt, steps = timed(len(idx), make_workload, seconds, 1000)
+    k_total, k_dt, k_steps = timed(64, make_kickoff, min(5.0, seconds / 3), 5000)
+    info = host_info()
     return {"value": total / dt, "unit": "env-steps/s", "cores": threads, "kind": "reference",
-            "sample": f"64 1v1 arenas x {steps} env steps (tick-skip 8, random default-table actions, "
-                      f"physics only) on {threads} threads, {dt:.1f} s"}
+            "sample": f"{len(idx)} arenas (every {sample_every}th of the 65536-arena {workload} workload, seed 4) x "
+                      f">= {steps} env steps, full env step (8 ticks, DefaultObs, 4 stub rewards, goal / no-touch "
+                      f"terminals, kickoff auto-reset), {threads} threads, {dt:.1f} s",
+            "arena_ticks_per_s": total * tick_skip / dt, "env_steps_per_s_per_core": total / dt / threads,
+            "host": info,
+            "c1_kickoff_64": {"value": k_total / k_dt, "unit": "env-steps/s",
+                              "sample": f"64 1v1 arenas from kickoffs x >= {k_steps} env steps, {threads} threads, "
+                                        f"{k_dt:.1f} s"}}
 
 
 def measured_traffic(workload):
@@ -270,7 +360,8 @@ def main():
     achieved = algo_bytes / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_seconds, max(1, min(16, os.cpu_count() or 1)))
+        cpu = cpu_baseline(args.cpu_seconds, host_info()["threads"],
+                           args.workload if args.workload != "mixed" else "random")
     wl = {"stress": "contact-heavy states (seed 4: 30% ball near corners/ramps/goal mouths, 20% cars on walls or "
                     "ceiling, 10% overlapping car spawns)",
           "random": "random field states (seed 4)",

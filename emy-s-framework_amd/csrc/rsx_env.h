@@ -129,8 +129,11 @@ RSX_HD void obs_row(const Arena<MAXC>& A, int self, const float (*prev)[kActionF
     int w = kObsHead;
     obs_player<MAXC>(me, inv, o + w);
     w += kPlayerObs;
-    for (int pass = 0; pass < 2; pass++) {  // teammates, then opponents, in slot order
-        for (int j = 0; j < A.ncars; j++) {
+    // Teammates, then opponents (DefaultObs.cpp:36-50), each in the order of
+    // GameState::players, which follows Arena::_cars: last-added car first
+    // (GameState.cpp:68-78), i.e. descending slot.
+    for (int pass = 0; pass < 2; pass++) {
+        for (int j = A.ncars - 1; j >= 0; j--) {
             if (j == self) continue;
             bool mate = A.car[j].team == me.team;
             if (mate != (pass == 0)) continue;

@@ -57,18 +57,19 @@ def random_field_states(batch, seed):
     nc = batch.num_cars
     pos = np.zeros((nc, 2))
     arena_of = np.repeat(np.arange(n), [2 * t for t in batch.team_size])
+    first = np.concatenate([[0], np.cumsum([2 * t for t in batch.team_size])[:-1]])
     todo = np.arange(nc)
     while todo.size:
         pos[todo] = np.stack([rng.uniform(-3900, 3900, todo.size), rng.uniform(-4900, 4900, todo.size)], 1)
         bad = np.linalg.norm(pos[todo] - bpos[arena_of[todo], :2], axis=1) < 300
-        # cars of one arena keep 300 uu apart (checked against earlier slots)
-        first = np.concatenate([[0], np.cumsum([2 * t for t in batch.team_size])[:-1]])
-        for i in todo[~bad]:
-            a = arena_of[i]
-            for j in range(first[a], i):
-                if np.linalg.norm(pos[i] - pos[j]) < 300:
-                    bad[np.searchsorted(todo, i)] = True
-                    break
+        # cars of one arena keep 300 uu apart (checked against the earlier
+        # slots' current positions)
+        for k in range(1, 6):
+            j = todo - k
+            ok = j >= first[arena_of[todo]]
+            near = np.zeros(todo.size, bool)
+            near[ok] = np.linalg.norm(pos[todo[ok]] - pos[j[ok]], axis=1) < 300
+            bad |= near
         todo = todo[bad]
     yaw = rng.uniform(-np.pi, np.pi, nc)
     cy, sy = np.cos(yaw), np.sin(yaw)

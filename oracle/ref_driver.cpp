@@ -585,3 +585,173 @@ extern "C" int ref_mesh_query(int mesh, const float* aabb6, int32_t* out, int ca
                                       btVector3(aabb6[3], aabb6[4], aabb6[5]));
     return cb.n;
 }
+
+// ---- The RLGym env half of a step over the oracle, for the CPU baseline ----
+// A C++ restatement of the pieces the engine's fused env step computes, run
+// on the oracle's own states: DefaultObs rows (DefaultObs.cpp:4-54 with
+// InvertPhys, StateUtil.cpp:3-17, and GameState's pad order,
+// GameState.cpp:93-109), the four stub rewards (GoalReward,
+// TouchBallReward, VelocityBallToGoalReward, VelocityPlayerToBallReward,
+// CommonRewards.h:33-99, weights 1 and concede -1), GoalScoreCondition and
+// NoTouchCondition (EnvSet.cpp:169-242), and the kickoff reset of terminal
+// arenas.  pad_map[i] is the arena pad at RLGym BOOST_LOCATIONS slot i.
+// tests/test_env_cpu.py checks it against oracle/rlgym_spec.py.
+namespace {
+
+struct EnvCtx {
+    int tick_skip;
+    const rsx_controls* lut;
+    int lut_n;
+    const int32_t* pad_map;
+    float timeout;
+    int auto_reset;
+};
+
+const Vec kInv(-1, -1, 1);
+
+void obs_player(const CarState& s, bool inv, float*& o) {
+    const Vec pos_coef(1 / 4096.f, 1 / 5120.f, 1 / 2044.f);
+    const float vel_coef = 1 / 2300.f, ang_coef = 1 / 5.5f;
+    Vec pos = s.pos, fwd = s.rotMat.forward, up = s.rotMat.up, vel = s.vel, ang = s.angVel;
+    if (inv) {
+        pos = pos * kInv; fwd = fwd * kInv; up = up * kInv; vel = vel * kInv; ang = ang * kInv;
+    }
+    Vec p = pos * pos_coef, v = vel * vel_coef, w = ang * ang_coef;
+    const float vals[19] = {p.x, p.y, p.z, fwd.x, fwd.y, fwd.z, up.x, up.y, up.z, v.x, v.y, v.z, w.x, w.y, w.z,
+                            s.boost / 100, (float)s.isOnGround, (float)s.HasFlipOrJump(), (float)s.isDemoed};
+    for (float x : vals) *o++ = x;
+}
+
+// One env step of arena h (cars in creation order, act[i] for car i).
+// Writes rew[i] and the DefaultObs row of car i at obs + i * row; returns the
+// terminal type (0 none, 1 goal, 2 no-touch truncation).
+int env_step_one(RefArena* h, const int32_t* act, const EnvCtx& c, float& no_touch, float* obs, float* rew,
+                 int reset_seed) {
+    const int n = (int)h->cars.size();
+    float prev[RSX_MAX_CARS][8];
+    for (int i = 0; i < n; i++) {
+        int k = act[i] < 0 ? 0 : (act[i] >= c.lut_n ? c.lut_n - 1 : act[i]);
+        const rsx_controls& ct = c.lut[k];
+        h->cars[i]->controls = to_controls(ct);
+        const float p[8] = {ct.throttle, ct.steer, ct.pitch, ct.yaw, ct.roll, (float)ct.jump, (float)ct.boost,
+                            (float)ct.handbrake};
+        memcpy(prev[i], p, sizeof(p));
+    }
+    Arena* a = h->arena;
+    a->Step(c.tick_skip);
+    const bool goal = a->IsBallScored();
+    const BallState ball = a->ball->GetState();
+    bool touched[RSX_MAX_CARS], any = false;
+    CarState cs[RSX_MAX_CARS];
+    for (int i = 0; i < n; i++) {
+        cs[i] = h->cars[i]->GetState();
+        touched[i] = cs[i].ballHitInfo.isValid &&
+                     cs[i].ballHitInfo.tickCountWhenHit >= a->tickCount - (uint64_t)c.tick_skip;
+        any |= touched[i];
+    }
+    int term = goal ? 1 : 0;
+    if (c.timeout > 0) {
+        if (any) {
+            no_touch = 0;
+        } else {
+            no_touch += c.tick_skip * (1 / 120.f);
+            if (no_touch >= c.timeout && term == 0) term = 2;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        const Team team = h->cars[i]->team;
+        float r = 0;
+        float g = 0;
+        if (goal) g = (team != RS_TEAM_FROM_Y(ball.pos.y)) ? 1.f : -1.f;
+        r += g * 1.f;
+        r += (touched[i] ? 1.f : 0.f) * 1.f;
+        const Vec target(0, team == Team::BLUE ? 6000.f : -6000.f, 642.775f / 2);
+        r += (target - ball.pos).Normalized().Dot(ball.vel / 6000.f) * 1.f;
+        r += (ball.pos - cs[i].pos).Normalized().Dot(cs[i].vel / 2300.f) * 1.f;
+        rew[i] = r;
+    }
+    if (term && c.auto_reset) {
+        a->ResetToRandomKickoff(reset_seed);
+        a->tickCount = 0;
+        no_touch = 0;
+        for (int i = 0; i < n; i++) {
+            cs[i] = h->cars[i]->GetState();
+            for (int k = 0; k < 8; k++) prev[i][k] = 0;
+        }
+    }
+    const BallState b = a->ball->GetState();
+    const auto& pads = a->GetBoostPads();
+    const int row = 51 + 19 * n;
+    for (int i = 0; i < n; i++) {
+        float* o = obs + (size_t)i * row;
+        const bool inv = h->cars[i]->team == Team::ORANGE;
+        const Vec pos_coef(1 / 4096.f, 1 / 5120.f, 1 / 2044.f);
+        Vec bp = b.pos, bv = b.vel, ba = b.angVel;
+        if (inv) { bp = bp * kInv; bv = bv * kInv; ba = ba * kInv; }
+        Vec p = bp * pos_coef, v = bv * (1 / 2300.f), w = ba * (1 / 5.5f);
+        const float head[9] = {p.x, p.y, p.z, v.x, v.y, v.z, w.x, w.y, w.z};
+        for (float x : head) *o++ = x;
+        for (int k = 0; k < 8; k++) *o++ = prev[i][k];
+        for (int k = 0; k < 34; k++) {
+            int slot = inv ? c.pad_map[33 - k] : c.pad_map[k];
+            *o++ = pads[slot]->GetState().isActive ? 1.f : 0.f;
+        }
+        obs_player(cs[i], inv, o);
+        // teammates then opponents, in GameState::players (Arena::_cars) order
+        for (int pass = 0; pass < 2; pass++)
+            for (int j = n - 1; j >= 0; j--) {
+                if (j == i) continue;
+                const bool mate = h->cars[j]->team == h->cars[i]->team;
+                if (mate == (pass == 0)) obs_player(cs[j], inv, o);
+            }
+    }
+    return term;
+}
+
+}  // namespace
+
+// One env step of each arena [arenas, arenas + n) with the given action
+// indices (the arenas' cars back to back); obs rows, rewards and terminals
+// out in the same order; no_touch[a] carries each arena's NoTouchCondition
+// timer.  Terminal arenas are reset to a kickoff drawn from reset_seed + a
+// when auto_reset.
+extern "C" void ref_env_step(void** arenas, int n, const int32_t* actions, int tick_skip, const rsx_controls* lut,
+                             int lut_n, const int32_t* pad_map, float timeout, int auto_reset, float* no_touch,
+                             float* obs, float* rew, uint8_t* term, int reset_seed) {
+    EnvCtx c{tick_skip, lut, lut_n, pad_map, timeout, auto_reset};
+    size_t car = 0, of = 0;
+    for (int a = 0; a < n; a++) {
+        auto* h = (RefArena*)arenas[a];
+        const int nc = (int)h->cars.size();
+        term[a] = (uint8_t)env_step_one(h, actions + car, c, no_touch[a], obs + of, rew + car, reset_seed + a);
+        car += nc;
+        of += (size_t)nc * (51 + 19 * nc);
+    }
+}
+
+// CPU baseline loop: `steps` env steps over the arenas, one uniform table
+// action per car per step from a xorshift32 stream seeded by `seed`, the env
+// outputs of each step written to scratch owned by the call.  Returns the
+// number of env steps done.
+extern "C" int64_t ref_env_run(void** arenas, int n, int steps, int tick_skip, uint32_t seed, const rsx_controls* lut,
+                               int lut_n, const int32_t* pad_map, float timeout) {
+    EnvCtx c{tick_skip, lut, lut_n, pad_map, timeout, 1};
+    uint32_t r = seed ? seed : 1u;
+    auto next = [&]() {
+        r ^= r << 13;
+        r ^= r >> 17;
+        r ^= r << 5;
+        return r;
+    };
+    std::vector<float> no_touch(n, 0.f), obs(RSX_MAX_CARS * (51 + 19 * RSX_MAX_CARS)), rew(RSX_MAX_CARS);
+    int32_t act[RSX_MAX_CARS];
+    int64_t done = 0;
+    for (int s = 0; s < steps; s++)
+        for (int a = 0; a < n; a++) {
+            auto* h = (RefArena*)arenas[a];
+            for (size_t i = 0; i < h->cars.size(); i++) act[i] = (int32_t)(next() % (uint32_t)lut_n);
+            env_step_one(h, act, c, no_touch[a], obs.data(), rew.data(), (int)(next() & 0x7fffffff));
+            done++;
+        }
+    return done;
+}

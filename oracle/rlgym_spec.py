@@ -9,7 +9,9 @@ that does not exist), so this restatement is pinned by closed forms and
 invariants in tests/test_spec_oracle.py instead of by a build of it.
 
 Inputs are the public records (CarState / BallState / BoostPadState ctypes
-structures, uu units) of one arena, in the arena's car order.
+structures, uu units) of one arena, in the arena's car order (the order cars
+were added).  GameState::players follows Arena::_cars, which RocketSim visits
+last-added first (the oracle's arenas pin that order, ref_car_order).
 
 Reference lines followed:
   DefaultObs::BuildObs / AddPlayerToObs   ObsBuilders/DefaultObs.cpp:4-54, DefaultObs.h:11-17
@@ -97,8 +99,11 @@ def default_obs(cars, ball, pads, prev_actions, self_idx):
     slots = PAD_MAP[::-1] if inv else PAD_MAP
     row.append(np.array([float(pads[j].is_active) for j in slots], dtype=F))
     row.append(player_block(me, inv))
-    mates = [player_block(c, inv) for j, c in enumerate(cars) if j != self_idx and c.team == me.team]
-    opps = [player_block(c, inv) for j, c in enumerate(cars) if j != self_idx and c.team != me.team]
+    # other players in GameState::players order, which is Arena::_cars order:
+    # last-added car first (GameState.cpp:68-78; DefaultObs.cpp:36-50)
+    others = [(j, c) for j, c in enumerate(cars) if j != self_idx][::-1]
+    mates = [player_block(c, inv) for j, c in others if c.team == me.team]
+    opps = [player_block(c, inv) for j, c in others if c.team != me.team]
     return np.concatenate(row + mates + opps).astype(F)
 
 

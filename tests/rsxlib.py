@@ -105,6 +105,11 @@ def load_oracle():
         ("ref_mesh_query", C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int]),
         ("ref_trace_contacts", None, [VP, C.c_int]),
         ("ref_trace_read", C.c_int, [C.POINTER(C.c_float), C.c_int]),
+        ("ref_env_step", None, [C.POINTER(VP), C.c_int, C.POINTER(C.c_int32), C.c_int, C.POINTER(Controls), C.c_int,
+                                C.POINTER(C.c_int32), C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_int]),
+        ("ref_env_run", C.c_int64, [C.POINTER(VP), C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(Controls),
+                                    C.c_int, C.POINTER(C.c_int32), C.c_float]),
     ])
     if lib is not None:
         assert lib.ref_init(MESH_DIR.encode()) == 0

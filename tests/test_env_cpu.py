@@ -163,3 +163,79 @@ def test_engine_touch_and_flip_boundaries(cpucheck):
         lib.cc_arena_free(h)
         assert obs[51 + 17] == want, t_air
         assert S.has_flip_or_jump(blue) == bool(want)
+
+
+def test_oracle_env_step_matches_spec(oracle):
+    """The oracle shim's C++ env step (ref_env_step: the CPU baseline's env
+    half) against the spec restatement on the oracle's own post-step records:
+    1v1, 2v2 and 3v3 arenas, 20 steps of random table actions, no-touch limit
+    0.5 s, then a goal with the kickoff auto-reset."""
+    lut = R.engine.default_action_lut()
+    lt = _lut()
+    pad_map = (C.c_int32 * 34)(*[int(x) for x in S.PAD_MAP])
+    teams = [1, 2, 3]
+    hs = []
+    for k, t in enumerate(teams):
+        h = oracle.ref_arena_new(t, t)
+        oracle.ref_reset_kickoff(C.c_void_p(h), 5 + k)
+        hs.append(h)
+    arr = (C.c_void_p * len(hs))(*hs)
+    ncars = [2 * t for t in teams]
+    rows = [51 + 19 * n for n in ncars]
+    nt = (C.c_float * len(hs))()
+    obs = (C.c_float * sum(n * r for n, r in zip(ncars, rows)))()
+    rew = (C.c_float * sum(ncars))()
+    term = (C.c_uint8 * len(hs))()
+    spec_nt = [np.float32(0)] * len(hs)
+    rng = np.random.default_rng(2)
+    seen = set()
+    try:
+        for step in range(20):
+            act = rng.integers(0, 90, sum(ncars)).astype(np.int32)
+            oracle.ref_env_step(arr, len(hs), act.ctypes.data_as(C.POINTER(C.c_int32)), 8, lut, len(lut), pad_map,
+                                C.c_float(0.5), 0, nt, obs, rew, term, 100)
+            car0 = of = 0
+            for a, h in enumerate(hs):
+                n = ncars[a]
+                cars = [R.CarState() for _ in range(n)]
+                for i in range(n):
+                    oracle.ref_get_car(C.c_void_p(h), i, C.byref(cars[i]))
+                ball = R.BallState()
+                oracle.ref_get_ball(C.c_void_p(h), C.byref(ball))
+                pads = R.PADS()
+                oracle.ref_get_pads(C.c_void_p(h), pads)
+                tick = oracle.ref_get_tick(C.c_void_p(h))
+                prev = np.array([lt[j] for j in act[car0:car0 + n]], dtype=np.float32)
+                touched = [S.touched_in_step(c, tick, 8) for c in cars]
+                goal = S.ball_scored(ball)
+                got = np.array(obs[of:of + n * rows[a]], dtype=np.float32).reshape(n, rows[a])
+                want = np.stack([S.default_obs(cars, ball, pads, prev, i) for i in range(n)])
+                np.testing.assert_array_equal(got, want, err_msg=f"arena {a} step {step}")
+                np.testing.assert_allclose(np.array(rew[car0:car0 + n]), S.stub_rewards(cars, ball, goal, touched, W),
+                                           atol=1e-6, err_msg=f"arena {a} step {step}")
+                t, spec_nt[a] = S.step_terminal(goal, any(touched), 8, spec_nt[a], 0.5)
+                assert term[a] == t and np.float32(nt[a]) == spec_nt[a], (a, step)
+                seen.add(int(t))
+                car0 += n
+                of += n * rows[a]
+        assert 2 in seen, seen
+        # a goal: rewards from the scoring state, then the kickoff and its obs
+        b = P.ball((0, 5150, 300), (0, 2500, 0))
+        oracle.ref_set_ball(C.c_void_p(hs[0]), C.byref(b))
+        act = np.zeros(sum(ncars), np.int32)
+        oracle.ref_env_step(arr, 1, act.ctypes.data_as(C.POINTER(C.c_int32)), 8, lut, len(lut), pad_map,
+                            C.c_float(0.5), 1, nt, obs, rew, term, 100)
+        assert term[0] == 1 and rew[0] > 0.5 and rew[1] < -0.5
+        cars = [R.CarState() for _ in range(2)]
+        for i in range(2):
+            oracle.ref_get_car(C.c_void_p(hs[0]), i, C.byref(cars[i]))
+        ball = R.BallState()
+        oracle.ref_get_ball(C.c_void_p(hs[0]), C.byref(ball))
+        pads = R.PADS()
+        oracle.ref_get_pads(C.c_void_p(hs[0]), pads)
+        assert abs(ball.pos.y) < 1 and oracle.ref_get_tick(C.c_void_p(hs[0])) == 0
+        want = np.stack([S.default_obs(cars, ball, pads, np.zeros((2, 8), np.float32), i) for i in range(2)])
+        np.testing.assert_array_equal(np.array(obs[:2 * 89], dtype=np.float32).reshape(2, 89), want)
+    finally:
+        for h in hs:
+            oracle.ref_arena_free(C.c_void_p(h))

@@ -167,3 +167,26 @@ def test_has_flip_or_jump_boundary():
     c.is_on_ground = 1
     c.air_time_since_jump = 5.0
     assert S.has_flip_or_jump(c)
+
+
+def test_other_players_follow_the_oracles_car_order(oracle):
+    """DefaultObs lists teammates, then opponents, in GameState::players
+    order (DefaultObs.cpp:36-50), which follows Arena::_cars: the order the
+    oracle's Arena::Step visits the cars (ref_car_order, last-added first)."""
+    for team in (2, 3):
+        h = C.c_void_p(oracle.ref_arena_new(team, team))
+        buf = (C.c_uint32 * 8)()
+        n = oracle.ref_car_order(h, buf, 8)
+        oracle.ref_arena_free(h)
+        order = [i - 1 for i in buf[:n]]  # car slots, visiting order
+        cars = [P.default_car(pos=(100.0 * (i + 1), -50.0 * i, 17), team=0 if i < team else 1) for i in range(n)]
+        ball = P.ball()
+        pads = R.PADS()
+        prev = np.zeros((n, 8), np.float32)
+        for me in range(n):
+            row = S.default_obs(cars, ball, pads, prev, me)
+            assert row.shape == (51 + 19 * n,)
+            xs = [round(float(row[51 + 19 * k]) * 4096 * (-1 if cars[me].team else 1)) for k in range(n)]
+            mates = [j for j in order if j != me and cars[j].team == cars[me].team]
+            opps = [j for j in order if cars[j].team != cars[me].team]
+            assert xs == [100 * (j + 1) for j in [me] + mates + opps], (team, me, xs)
