@@ -10,6 +10,31 @@
 //
 // Matrices are Bullet bases (rows); the detector's "R[i*4+j]" element is row
 // i, column j, and its column dot products keep the x, y, z summation order.
+//
+// This file is an altered restatement of zlib-licensed upstream code, kept
+// with its notice as that licence requires (the alterations: one GPU lane per
+// query, explicit stacks and index-linked pools instead of recursion and
+// pointers, the engine's own types):
+//
+//   Box-Box collision detection re-distributed under the ZLib license with
+//   permission from Russell L. Smith.  Original version is from Open Dynamics
+//   Engine, Copyright (C) 2001,2002 Russell L. Smith.  All rights reserved.
+//   Bullet Continuous Collision Detection and Physics Library, Bullet is
+//   Copyright (c) 2003-2006 Erwin Coumans  https://bulletphysics.org
+//
+//   This software is provided 'as-is', without any express or implied
+//   warranty.  In no event will the authors be held liable for any damages
+//   arising from the use of this software.  Permission is granted to anyone
+//   to use this software for any purpose, including commercial applications,
+//   and to alter it and redistribute it freely, subject to the following
+//   restrictions:
+//   1. The origin of this software must not be misrepresented; you must not
+//      claim that you wrote the original software.  If you use this software
+//      in a product, an acknowledgment in the product documentation would be
+//      appreciated but is not required.
+//   2. Altered source versions must be plainly marked as such, and must not
+//      be misrepresented as being the original software.
+//   3. This notice may not be removed or altered from any source distribution.
 #pragma once
 
 #include "rsx_math.h"

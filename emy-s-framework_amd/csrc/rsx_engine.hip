@@ -681,6 +681,13 @@ rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int
 rsx_status rsx_upload_action_lut(rsx_engine* e, const rsx_controls* lut, int32_t count) {
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (!lut || count <= 0 || count > e->lut_cap) return fail(RSX_INVALID_ARG, "action table must hold 1..1024 entries");
+    if (e->masks_on) {
+        // the DefaultAction masks index the default table's entries
+        const std::vector<rsx_controls> def = default_action_lut();
+        if (count != (int32_t)def.size() || memcmp(lut, def.data(), def.size() * sizeof(rsx_controls)) != 0)
+            return fail(RSX_INVALID_ARG, "action masks are on: only the default 90-entry table can be uploaded "
+                                         "(turn the masks off first)");
+    }
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipMemcpy(e->d_lut, lut, sizeof(rsx_controls) * count, hipMemcpyHostToDevice));
     e->lut_n = count;
@@ -1068,7 +1075,9 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
     bufs.push_back({e->d_rew, sizeof(float) * std::max(1, e->num_cars)});
     bufs.push_back({e->d_term, std::max<size_t>(1, (size_t)e->num_arenas)});
     if (e->cull.enabled) bufs.push_back({e->cull.stats, 3 * sizeof(unsigned long long)});
+    if (e->masks_on) bufs.push_back({e->d_masks, (size_t)e->num_cars * kNumDefaultActions});
     std::vector<void*> saved(bufs.size(), nullptr);
+    bool saved_all = false;  // every snapshot taken: restore() is safe on any exit
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     rsx_status status = RSX_OK;
@@ -1094,6 +1103,7 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
             HIP_TRY(hipMalloc(&saved[i], bufs[i].second));
             HIP_TRY(hipMemcpyAsync(saved[i], bufs[i].first, bufs[i].second, hipMemcpyDeviceToDevice, st));
         }
+        saved_all = true;
         int best = -1;
         for (size_t k = 0; k < res.size(); k++) {
             const LaunchShape s{res[k].tuning.block_lanes, res[k].tuning.arenas_per_wave,
@@ -1125,6 +1135,12 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
         return RSX_OK;
     };
     status = run();
+    if (status != RSX_OK && saved_all) {
+        // a failed trial leaves the arenas advanced: put them back before
+        // reporting the failure (best effort; the first error is the one kept)
+        (void)restore();
+        (void)hipStreamSynchronize(st);
+    }
     cleanup();
     if (status != RSX_OK) return status;
     if (results) std::copy(res.begin(), res.end(), results);

@@ -15,6 +15,29 @@
 // written here as bounded loops with explicit stacks, and its pointer-linked
 // face and vertex pools as index-linked arrays, so the code runs in one GPU
 // lane with a fixed private-memory frame.
+//
+// This file is an altered restatement of zlib-licensed upstream code, kept
+// with its notice as that licence requires (the alterations: one GPU lane per
+// query, explicit stacks and index-linked pools instead of recursion and
+// pointers, the engine's own types):
+//
+//   Bullet Continuous Collision Detection and Physics Library
+//   Copyright (c) 2003-2008 Erwin Coumans  https://bulletphysics.org
+//   GJK-EPA collision solver by Nathanael Presson, 2008
+//
+//   This software is provided 'as-is', without any express or implied
+//   warranty.  In no event will the authors be held liable for any damages
+//   arising from the use of this software.  Permission is granted to anyone
+//   to use this software for any purpose, including commercial applications,
+//   and to alter it and redistribute it freely, subject to the following
+//   restrictions:
+//   1. The origin of this software must not be misrepresented; you must not
+//      claim that you wrote the original software.  If you use this software
+//      in a product, an acknowledgment in the product documentation would be
+//      appreciated but is not required.
+//   2. Altered source versions must be plainly marked as such, and must not
+//      be misrepresented as being the original software.
+//   3. This notice may not be removed or altered from any source distribution.
 #pragma once
 
 #include "rsx_math.h"

@@ -192,6 +192,10 @@ def test_action_masks_match_spec(torch_cuda):
     recs = b.GetAllCarStates()
     assert all(list(got[k]) == S.action_mask(recs[k], masks) for k in range(b.num_cars))
     assert seen_roof
+    # with masks on, only the default table may be uploaded (the masks index it)
+    with pytest.raises(E.RsxError):
+        b.UploadDiscreteActionLut((E.CarControls * 40)(*E.default_action_lut()[:40]))
+    b.UploadDiscreteActionLut(E.default_action_lut())
     b.SetActionMasks(False)
     b.UploadDiscreteActionLut((E.CarControls * 40)(*E.default_action_lut()[:40]))
     with pytest.raises(E.RsxError):

@@ -230,9 +230,11 @@ def test_autotune_restores_state_and_keeps_the_fastest(torch_cuda):
         return b
 
     b = make()
+    b.SetActionMasks(True)
     d = torch.tensor(np.random.default_rng(1).integers(0, 90, b.num_cars).astype(np.int32), device="cuda")
     b.StepEnv(d.data_ptr())
     before = _snapshot(b)
+    masks_before = bytes(b.CopyBuiltDefaultActionMasks())
     res = b.Autotune(d.data_ptr(), env_steps=1)
     assert len(res) == 24
     feas = [r for r in res if r[3] is not None]
@@ -240,6 +242,7 @@ def test_autotune_restores_state_and_keeps_the_fastest(torch_cuda):
     best = min(feas, key=lambda r: r[3])
     assert b.GetTuning() == best[:3]
     assert _snapshot(b) == before
+    assert bytes(b.CopyBuiltDefaultActionMasks()) == masks_before  # the trials' masks are rolled back too
     b.StepEnv(d.data_ptr())
     got = _snapshot(b)
     b.Destroy()
