@@ -253,10 +253,11 @@ class RocketSimBatch:
 
     # --- env half of the step -------------------------------------------------
     def ConfigureEnv(self, goal=1.0, concede=-1.0, touch=1.0, vel_ball_to_goal=1.0, vel_player_to_ball=1.0,
-                     no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True):
-        """ConfigureTrainingRewards + ConfigureTrainingTerminals."""
+                     no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True, obs_builder=0):
+        """ConfigureTrainingRewards + ConfigureTrainingTerminals; obs_builder
+        0 (DefaultObs) or 1 (AdvancedObs, BuildAdvancedObsAndDefaultMasks)."""
         c = EnvConfig(goal, concede, touch, vel_ball_to_goal, vel_player_to_ball, no_touch_timeout_s,
-                      int(goal_terminal), int(auto_reset))
+                      int(goal_terminal), int(auto_reset), int(obs_builder))
         _check(self._lib.rsx_configure_env(self._h, C.byref(c)))
 
     def StepEnv(self, device_actions_ptr, stream=None):

@@ -155,9 +155,9 @@ __global__ void __launch_bounds__(64) k_env_obs(uint32_t* words, int n, CarGeom 
     float prev[MAXC][kActionFloats];
     for (int i = 0; i < MAXC; i++)
         for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
-    const int row = obs_row_len(A.ncars);
+    const int row = obs_row_len_of(E.obs_kind, A.ncars);
     float* o = obs + obs_off[a];
-    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    for (int i = 0; i < A.ncars; i++) env_obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
     if (masks)
         for (int i = 0; i < A.ncars; i++)
             action_mask_row(A.car[i], mask_tables, masks + (int64_t)(car_off[a] + i) * kNumDefaultActions);
@@ -183,6 +183,22 @@ rsx_status upload(T** dst, const std::vector<T>& src) {
 }
 
 int bucket_of_team(int t) { return t - 1; }
+
+// Obs rows arena-major, one per car, for observation builder `kind`.
+void obs_layout_for(rsx_engine* e, int kind) {
+    e->car_obs_off.resize(e->num_cars);
+    e->car_obs_len.resize(e->num_cars);
+    int64_t off = 0;
+    for (int a = 0; a < e->num_arenas; a++) {
+        const int nc = 2 * e->arena_team[a];
+        for (int i = 0; i < nc; i++) {
+            e->car_obs_off[e->arena_car_off[a] + i] = off;
+            e->car_obs_len[e->arena_car_off[a] + i] = obs_row_len_of(kind, nc);
+            off += obs_row_len_of(kind, nc);
+        }
+    }
+    e->obs_floats = off;
+}
 
 // Host-side pack/unpack of one arena through a contiguous staging column.
 template <int MAXC>
@@ -399,19 +415,13 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         E.no_touch_timeout = cfg->no_touch_timeout_s;
         E.goal_terminal = 1;
         E.auto_reset = 1;
+        E.obs_kind = kObsDefault;
         if (!obs_pad_map(e->hw.pads, E.pad_map)) return fail(RSX_MESH_LOAD_FAILED, "boost pad layout does not match");
-        e->car_obs_off.resize(e->num_cars);
-        e->car_obs_len.resize(e->num_cars);
-        int64_t off = 0;
-        for (int a = 0; a < e->num_arenas; a++) {
-            int nc = 2 * e->arena_team[a];
-            for (int i = 0; i < nc; i++) {
-                e->car_obs_off[e->arena_car_off[a] + i] = off;
-                e->car_obs_len[e->arena_car_off[a] + i] = obs_row_len(nc);
-                off += obs_row_len(nc);
-            }
-        }
-        e->obs_floats = off;
+        // the obs buffer holds the longer (AdvancedObs) rows, so its address
+        // survives a change of observation builder
+        obs_layout_for(e.get(), kObsAdvanced);
+        const int64_t off = e->obs_floats;
+        obs_layout_for(e.get(), kObsDefault);
         HIP_TRY(hipMalloc((void**)&e->d_obs, sizeof(float) * std::max<int64_t>(1, off)));
         HIP_TRY(hipMalloc((void**)&e->d_rew, sizeof(float) * (size_t)e->num_cars));
         HIP_TRY(hipMalloc((void**)&e->d_term, (size_t)e->num_arenas));
@@ -891,6 +901,21 @@ rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c) {
     E.no_touch_timeout = c->no_touch_timeout_s;
     E.goal_terminal = c->goal_terminal != 0;
     E.auto_reset = c->auto_reset != 0;
+    if (c->obs_builder != RSX_OBS_DEFAULT && c->obs_builder != RSX_OBS_ADVANCED)
+        return fail(RSX_INVALID_ARG, "obs_builder must be RSX_OBS_DEFAULT or RSX_OBS_ADVANCED");
+    if (c->obs_builder != E.obs_kind) {
+        HIP_TRY(hipSetDevice(e->device));
+        HIP_TRY(hipDeviceSynchronize());  // no env step may still read the old offsets
+        E.obs_kind = c->obs_builder;
+        obs_layout_for(e, E.obs_kind);
+        for (int k = 0; k < 3; k++) {
+            Bucket& b = e->bucket[k];
+            if (!b.n) continue;
+            std::vector<int64_t> ooff(b.n);
+            for (int l = 0; l < b.n; l++) ooff[l] = e->car_obs_off[e->arena_car_off[b.global[l]]];
+            HIP_TRY(hipMemcpy(b.d_obs_off, ooff.data(), sizeof(int64_t) * b.n, hipMemcpyHostToDevice));
+        }
+    }
     e->gen++;
     return RSX_OK;
 }
@@ -906,6 +931,7 @@ rsx_status rsx_get_env_config(const rsx_engine* e, rsx_env_config* c) {
     c->no_touch_timeout_s = E.no_touch_timeout;
     c->goal_terminal = E.goal_terminal;
     c->auto_reset = E.auto_reset;
+    c->obs_builder = E.obs_kind;
     return RSX_OK;
 }
 

@@ -28,6 +28,14 @@ constexpr int kObsHead = 9 + kActionFloats + kNumPads;  // ball, previous action
 
 RSX_HD int obs_row_len(int ncars) { return kObsHead + kPlayerObs * ncars; }
 
+// Observation builders of the env step.
+constexpr int kObsDefault = 0;   // RLGym DefaultObs: 51 + 19 per car
+constexpr int kObsAdvanced = 1;  // GigaLearn AdvancedObs as the GPU path builds it: 51 + 29 per car
+constexpr int kAdvPlayerObs = 29;
+RSX_HD int obs_row_len_of(int kind, int ncars) {
+    return kObsHead + (kind == kObsAdvanced ? kAdvPlayerObs : kPlayerObs) * ncars;
+}
+
 // Terminal codes (TerminalCondition.h TerminalType)
 constexpr uint8_t kNotTerminal = 0, kTerminalNormal = 1, kTerminalTruncated = 2;
 
@@ -36,6 +44,7 @@ struct EnvConfig {
     float no_touch_timeout;   // seconds; <= 0 disables NoTouchCondition
     int goal_terminal;        // GoalScoreCondition enabled
     int auto_reset;           // kickoff reset of terminal arenas inside the step
+    int obs_kind;             // kObsDefault or kObsAdvanced
     int8_t pad_map[kNumPads]; // obs pad slot i -> engine pad index (GameState boostPadIndexMap)
 };
 
@@ -141,6 +150,86 @@ RSX_HD void obs_row(const Arena<MAXC>& A, int self, const float (*prev)[kActionF
             w += kPlayerObs;
         }
     }
+}
+
+// ---- AdvancedObs ----------------------------------------------------------
+// The ContinuousV2 + attention observation of the reference's GPU training
+// path (RocketSimCuda/src/TrainingObs.cuh:20-127, consumed through
+// buildAdvancedObsAndDefaultMasksKernel, RocketSimCudaObs.cuh:13-64), whose
+// fields are GigaLearn's AdvancedObs (AdvancedObs.cpp:4-65): per row the ball
+// (position / 5000, velocity / 2300, angular velocity / 3, orange viewers
+// mirrored in x and y), the car's last controls, per pad 1 when active else
+// 1 / (1 + cooldown) (same pad order and mirror as DefaultObs), then 29 floats
+// for the car itself, its teammates and its opponents, each in slot order.
+// Where the CPU AdvancedObs differs (its pad timers come from the mirrored
+// pad, GameState.cpp:64-66, and it lists other players in Arena::_cars
+// order), the GPU path's layout is the one restated.
+RSX_HD V3 obs_inv(V3 v, bool inv) { return inv ? V3{-v.x, -v.y, v.z} : v; }
+
+RSX_HD void adv_obs_player(const Car& c, bool inv, V3 ball_pos, V3 ball_vel, float* o) {
+    const float pc = 1.f / 5000.f, vc = 1.f / 2300.f, ac = 1.f / 3.f;
+    const V3 pos = obs_inv(c.xf.o * kBT2UU, inv), vel = obs_inv(c.vel * kBT2UU, inv), ang = obs_inv(c.ang, inv);
+    const V3 f = obs_inv(col(c.xf.b, 0), inv), r = obs_inv(col(c.xf.b, 1), inv), u = obs_inv(col(c.xf.b, 2), inv);
+    // rotmat_dot_vec: the vector in the car's (forward, right, up) frame
+    auto local = [&](V3 v) { return V3{rl_dot(f, v), rl_dot(r, v), rl_dot(u, v)}; };
+    const V3 la = local(ang), lp = local(ball_pos - pos), lv = local(ball_vel - vel);
+    const float vals[kAdvPlayerObs] = {pos.x * pc, pos.y * pc, pos.z * pc, f.x, f.y, f.z, u.x, u.y, u.z,
+                                       vel.x * vc, vel.y * vc, vel.z * vc, ang.x * ac, ang.y * ac, ang.z * ac,
+                                       la.x * ac, la.y * ac, la.z * ac, lp.x * pc, lp.y * pc, lp.z * pc,
+                                       lv.x * vc, lv.y * vc, lv.z * vc, c.boost / 100.f, c.on_ground ? 1.f : 0.f,
+                                       has_flip_or_jump(c) ? 1.f : 0.f, c.is_demoed ? 1.f : 0.f,
+                                       c.has_jumped ? 1.f : 0.f};
+    for (int k = 0; k < kAdvPlayerObs; k++) o[k] = vals[k];
+}
+
+// AdvancedObs row of car `self`: obs_row_len_of(kObsAdvanced, ncars) floats.
+template <int MAXC>
+RSX_HD void adv_obs_row(const Arena<MAXC>& A, int self, const EnvConfig& E, float* o) {
+    const Car& me = A.car[self];
+    const bool inv = me.team != 0;
+    const float pc = 1.f / 5000.f, vc = 1.f / 2300.f, ac = 1.f / 3.f;
+    const V3 bp = obs_inv(A.ball.xf.o * kBT2UU, inv), bv = obs_inv(A.ball.vel * kBT2UU, inv);
+    const V3 ba = obs_inv(A.ball.ang, inv);
+    o[0] = bp.x * pc;
+    o[1] = bp.y * pc;
+    o[2] = bp.z * pc;
+    o[3] = bv.x * vc;
+    o[4] = bv.y * vc;
+    o[5] = bv.z * vc;
+    o[6] = ba.x * ac;
+    o[7] = ba.y * ac;
+    o[8] = ba.z * ac;
+    const Controls& lc = me.last_controls;
+    o[9] = lc.throttle;
+    o[10] = lc.steer;
+    o[11] = lc.pitch;
+    o[12] = lc.yaw;
+    o[13] = lc.roll;
+    o[14] = lc.jump ? 1.f : 0.f;
+    o[15] = lc.boost ? 1.f : 0.f;
+    o[16] = lc.handbrake ? 1.f : 0.f;
+    for (int i = 0; i < kNumPads; i++) {
+        const Pad& pad = A.pad[inv ? E.pad_map[kNumPads - 1 - i] : E.pad_map[i]];
+        o[9 + kActionFloats + i] = pad.is_active ? 1.f : (1.f / (1.f + pad.cooldown));
+    }
+    int w = kObsHead;
+    adv_obs_player(me, inv, bp, bv, o + w);
+    w += kAdvPlayerObs;
+    for (int pass = 0; pass < 2; pass++)  // teammates, then opponents, in slot order
+        for (int j = 0; j < A.ncars; j++) {
+            if (j == self || (A.car[j].team == me.team) != (pass == 0)) continue;
+            adv_obs_player(A.car[j], inv, bp, bv, o + w);
+            w += kAdvPlayerObs;
+        }
+}
+
+// The configured observation row of car `self`; prev is DefaultObs's
+// previous-action block (AdvancedObs reads the car's last controls instead).
+template <int MAXC>
+RSX_HD void env_obs_row(const Arena<MAXC>& A, int self, const float (*prev)[kActionFloats], const EnvConfig& E,
+                        float* o) {
+    if (E.obs_kind == kObsAdvanced) adv_obs_row<MAXC>(A, self, E, o);
+    else obs_row<MAXC>(A, self, prev, E, o);
 }
 
 RSX_HD bool touched_in_step(const Car& c, uint64_t tick, int tick_skip) {

@@ -143,9 +143,9 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
         for (int i = 0; i < A.ncars; i++)
             for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;
     }
-    const int row = obs_row_len(A.ncars);
+    const int row = obs_row_len_of(E.obs_kind, A.ncars);
     float* o = V.obs + B.obs_off[a];
-    for (int i = 0; i < A.ncars; i++) obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
+    for (int i = 0; i < A.ncars; i++) env_obs_row<MAXC>(A, i, prev, E, o + (int64_t)i * row);
     if (V.masks)
         for (int i = 0; i < A.ncars; i++)
             action_mask_row(A.car[i], V.mask_tables, V.masks + (int64_t)(c0 + i) * kNumDefaultActions);

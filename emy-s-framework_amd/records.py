@@ -78,7 +78,12 @@ class Config(C.Structure):
 class EnvConfig(C.Structure):
     _fields_ = [("goal_weight", C.c_float), ("concede_scale", C.c_float), ("touch_weight", C.c_float),
                 ("vel_ball_to_goal_weight", C.c_float), ("vel_player_to_ball_weight", C.c_float),
-                ("no_touch_timeout_s", C.c_float), ("goal_terminal", C.c_int32), ("auto_reset", C.c_int32)]
+                ("no_touch_timeout_s", C.c_float), ("goal_terminal", C.c_int32), ("auto_reset", C.c_int32),
+                ("obs_builder", C.c_int32)]
+
+
+OBS_DEFAULT = 0
+OBS_ADVANCED = 1
 
 
 CARS_BLUE_FIRST = 0

@@ -133,7 +133,18 @@ typedef struct rsx_env_config {
     float no_touch_timeout_s;          /* NoTouchCondition(maxTime); <= 0 off; default rsx_config's */
     int32_t goal_terminal;             /* GoalScoreCondition on; default 1 */
     int32_t auto_reset;                /* kickoff reset of terminal arenas inside rsx_step_env; default 1 */
+    int32_t obs_builder;               /* RSX_OBS_DEFAULT (default) or RSX_OBS_ADVANCED */
 } rsx_env_config;
+
+/* Observation builders.  RSX_OBS_DEFAULT: RLGym DefaultObs (DefaultObs.cpp:
+ * 4-54), 51 + 19 * cars floats per car, the previous action being the step's
+ * action.  RSX_OBS_ADVANCED: the AdvancedObs rows of the reference's GPU
+ * training path (BuildAdvancedObsAndDefaultMasks, RocketSimCuda.h:316;
+ * TrainingObs.cuh:20-127), 51 + 29 * cars floats per car (109 / 167 / 225),
+ * the previous action being the car's last controls.  Changing the builder
+ * changes rsx_obs_layout and obs_floats (the obs buffer's address stays). */
+#define RSX_OBS_DEFAULT 0
+#define RSX_OBS_ADVANCED 1
 
 /* ConfigureTrainingRewards + ConfigureTrainingTerminals (RocketSimCuda.h:329-330). */
 rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c);
@@ -155,7 +166,8 @@ rsx_status rsx_build_obs(rsx_engine* e, void* stream);
 
 /* Device pointers of the env outputs (GetBuilt*DevicePtr, RocketSimCuda.h:
  * 393-396): obs (obs_floats floats; car c's row at row_offset[c], length
- * row_len[c] = 51 + 19 * cars in its arena: 89 / 127 / 165), rewards
+ * row_len[c] = 51 + 19 * cars in its arena: 89 / 127 / 165 with DefaultObs,
+ * 51 + 29 * cars: 109 / 167 / 225 with AdvancedObs), rewards
  * (rsx_num_cars floats), terminals (N bytes).  Stable for the engine's life. */
 rsx_status rsx_env_buffers(rsx_engine* e, float** obs, float** rewards, uint8_t** terminals, int64_t* obs_floats);
 rsx_status rsx_obs_layout(const rsx_engine* e, int64_t* row_offset, int32_t* row_len, int32_t count);

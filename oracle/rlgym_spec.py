@@ -24,6 +24,8 @@ Reference lines followed:
   GoalScoreCondition / NoTouchCondition   TerminalConditions/GoalScoreCondition.h, NoTouchCondition.h
   reward sum and terminal priority        EnvSet/EnvSet.cpp:169-242
   DefaultAction masks                     ActionParsers/DefaultAction.cpp:60-118
+  AdvancedObs (GPU training layout)       RocketSimCuda/src/TrainingObs.cuh:20-127, fields of
+                                          ObsBuilders/AdvancedObs.cpp:4-65
 """
 import numpy as np
 
@@ -195,3 +197,53 @@ def action_mask(c, masks):
     if has_flip_or_jump(c) or turtled:
         out = [m | jm for m, jm in zip(out, jump)]
     return out
+
+
+ADV_POS, ADV_VEL, ADV_ANG = F(1) / F(5000), F(1) / F(2300), F(1) / F(3)
+
+
+def _inv(v, inv):
+    return (v * INV).astype(F) if inv else v
+
+
+def _dot3(a, b):
+    """float32 x*x' + y*y' + z*z', summed left to right (v3_dot)."""
+    return F(F(F(a[0] * b[0]) + F(a[1] * b[1])) + F(a[2] * b[2]))
+
+
+def _rot_dot(fwd, right, up, v):
+    """rotmat_dot_vec: v in the (forward, right, up) frame, float32."""
+    return np.array([_dot3(fwd, v), _dot3(right, v), _dot3(up, v)], dtype=F)
+
+
+def adv_player_block(c, inv, ball_pos, ball_vel):
+    """The 29 floats of one car (add_advanced_obs_player, TrainingObs.cuh:52-79)."""
+    pos, vel, ang = _inv(_v(c.pos), inv), _inv(_v(c.vel), inv), _inv(_v(c.ang_vel), inv)
+    fwd, right, up = _inv(_v(c.rot.forward), inv), _inv(_v(c.rot.right), inv), _inv(_v(c.rot.up), inv)
+    return np.concatenate([pos * ADV_POS, fwd, up, vel * ADV_VEL, ang * ADV_ANG,
+                           _rot_dot(fwd, right, up, ang) * ADV_ANG,
+                           _rot_dot(fwd, right, up, (ball_pos - pos).astype(F)) * ADV_POS,
+                           _rot_dot(fwd, right, up, (ball_vel - vel).astype(F)) * ADV_VEL,
+                           np.array([F(c.boost) / F(100), float(c.is_on_ground), float(has_flip_or_jump(c)),
+                                     float(c.is_demoed), float(c.has_jumped)], dtype=F)]).astype(F)
+
+
+def advanced_obs(cars, ball, pads, self_idx):
+    """AdvancedObs row of car self_idx as the GPU training path builds it
+    (device_build_advanced_obs, TrainingObs.cuh:85-125): ball, the car's last
+    controls, per pad 1 when active else 1 / (1 + cooldown), then the car,
+    its teammates and its opponents in slot order.  51 + 29 * cars floats."""
+    me = cars[self_idx]
+    inv = me.team != 0
+    bp, bv, ba = _inv(_v(ball.pos), inv), _inv(_v(ball.vel), inv), _inv(_v(ball.ang_vel), inv)
+    lc = me.last_controls
+    row = [bp * ADV_POS, bv * ADV_VEL, ba * ADV_ANG,
+           np.array([lc.throttle, lc.steer, lc.pitch, lc.yaw, lc.roll, float(bool(lc.jump)), float(bool(lc.boost)),
+                     float(bool(lc.handbrake))], dtype=F)]
+    slots = PAD_MAP[::-1] if inv else PAD_MAP
+    row.append(np.array([F(1) if pads[j].is_active else F(F(1) / (F(1) + F(pads[j].cooldown))) for j in slots],
+                        dtype=F))
+    row.append(adv_player_block(me, inv, bp, bv))
+    row += [adv_player_block(c, inv, bp, bv) for j, c in enumerate(cars) if j != self_idx and c.team == me.team]
+    row += [adv_player_block(c, inv, bp, bv) for j, c in enumerate(cars) if j != self_idx and c.team != me.team]
+    return np.concatenate(row).astype(F)

@@ -274,8 +274,8 @@ extern "C" void cc_ray_convex(int kind, const float* dims, const float* xf12, co
 // default stub-reward weights (1, concede -1, 1, 1, 1): each car's DefaultObs
 // row into obs (rows back to back), rewards, and the terminal type (return
 // value) with the no-touch timer carried through *no_touch.
-extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, float* no_touch, float* obs,
-                      float* rew) {
+extern "C" int cc_env_kind(void* p, const float* prev, int tick_skip, float timeout, float* no_touch, float* obs,
+                           float* rew, int obs_kind) {
     auto* a = (CcArena*)p;
     EnvConfig E;
     E.goal_w = 1.f;
@@ -286,6 +286,7 @@ extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, 
     E.no_touch_timeout = timeout;
     E.goal_terminal = 1;
     E.auto_reset = 0;
+    E.obs_kind = obs_kind;
     if (!obs_pad_map(a->world->w.pads, E.pad_map)) return -1;
     const Arena<6>& A = a->A;
     float pa[6][kActionFloats];
@@ -300,9 +301,15 @@ extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, 
     }
     uint8_t t = step_terminal(goal, any, (float)tick_skip * (1.f / kTickRate), *no_touch, E);
     for (int i = 0; i < A.ncars; i++) rew[i] = stub_reward<6>(A, i, goal, touched[i], E);
-    int row = obs_row_len(A.ncars);
-    for (int i = 0; i < A.ncars; i++) obs_row<6>(A, i, pa, E, obs + i * row);
+    int row = obs_row_len_of(obs_kind, A.ncars);
+    for (int i = 0; i < A.ncars; i++) env_obs_row<6>(A, i, pa, E, obs + i * row);
     return t;
+}
+
+// cc_env with DefaultObs rows.
+extern "C" int cc_env(void* p, const float* prev, int tick_skip, float timeout, float* no_touch, float* obs,
+                      float* rew) {
+    return cc_env_kind(p, prev, tick_skip, timeout, no_touch, obs, rew, kObsDefault);
 }
 
 // Rank of every triangle within its mesh's BVH traversal order (the engine's

@@ -140,6 +140,8 @@ def load_cpucheck():
         ("cc_ray_convex", None, [C.c_int] + [C.POINTER(C.c_float)] * 5),
         ("cc_tri_info", C.c_int, [VP, C.POINTER(C.c_float), C.c_int]),
         ("cc_contact_overflow", C.c_int, [VP]),
+        ("cc_env_kind", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int]),
         ("cc_tri_rank", C.c_int, [VP, C.POINTER(C.c_int32), C.c_int]),
         ("cc_point_tri_dist2", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
         ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),

@@ -239,3 +239,48 @@ def test_oracle_env_step_matches_spec(oracle):
     finally:
         for h in hs:
             oracle.ref_arena_free(C.c_void_p(h))
+
+
+def test_engine_advanced_obs_matches_spec(cpucheck):
+    """The engine's AdvancedObs rows (rsx_env.h adv_obs_row, host build)
+    against the spec restatement on the engine's own post-step records, for
+    1v1, 2v2 and 3v3 arenas of the mixed workload after 8 ticks of random
+    table actions."""
+    import workload_harness as WH
+    lib, world = cpucheck
+    lut = R.engine.default_action_lut()
+    cases = WH.sample("mixed", 24)
+    rng = np.random.default_rng(4)
+    seen = set()
+    for case in cases:
+        n = len(case["cars"])
+        h = C.c_void_p(lib.cc_arena_new(world, case["blue"], case["orange"]))
+        lib.cc_set_tick(h, case["tick"])
+        for i, s in enumerate(case["cars"]):
+            lib.cc_set_car(h, i, C.byref(s))
+            lib.cc_set_wheels(h, i, P.WHEEL_ZERO6)
+        lib.cc_set_ball(h, C.byref(case["ball"]))
+        lib.cc_set_pads(h, case["pads"])
+        for i, k in enumerate(rng.integers(0, 90, n)):
+            lib.cc_set_controls(h, i, C.byref(lut[k]))
+        lib.cc_step(h, 8)
+        row = 51 + 29 * n
+        obs = (C.c_float * (n * row))()
+        rew = (C.c_float * n)()
+        nt = C.c_float(0)
+        lib.cc_env_kind(h, (C.c_float * (8 * n))(), 8, 0.0, C.byref(nt), obs, rew, 1)
+        cars = [R.CarState() for _ in range(n)]
+        for i in range(n):
+            lib.cc_get_car(h, i, C.byref(cars[i]))
+        ball = R.BallState()
+        lib.cc_get_ball(h, C.byref(ball))
+        pads = R.PADS()
+        lib.cc_get_pads(h, pads)
+        lib.cc_arena_free(h)
+        got = np.array(obs, dtype=np.float32).reshape(n, row)
+        for i in range(n):
+            np.testing.assert_allclose(got[i], S.advanced_obs(cars, ball, pads, i), atol=2e-6,
+                                       err_msg=f"arena {case['source']} car {i}")
+        seen.add(n)
+        assert any(not p.is_active for p in pads)  # the pad timer branch is exercised
+    assert seen == {2, 4, 6}, seen

@@ -201,3 +201,51 @@ def test_action_masks_match_spec(torch_cuda):
     with pytest.raises(E.RsxError):
         b.SetActionMasks(True)
     b.Destroy()
+
+
+def test_advanced_obs_matches_spec(torch_cuda):
+    """The env step with the AdvancedObs builder (BuildAdvancedObsAndDefault-
+    Masks' layout): every row of a mixed 1v1/2v2/3v3 batch equals the spec
+    restatement on the post-step records, for several steps and after
+    rsx_build_obs; the layout has 51 + 29 floats per car and switching back
+    restores DefaultObs rows."""
+    import torch
+    from emy_s_framework_amd import workloads as WL
+    n = 90
+    b = E.RocketSimBatch(n, [1, 2, 3] * (n // 3), seed=6)
+    WL.random_field_states(b, 6)
+    b.ConfigureEnv(no_touch_timeout_s=0.0, auto_reset=True, obs_builder=E.OBS_ADVANCED)
+    off, ln = b.ObsLayout()
+    teams = [b.team_size[a] for a in range(n) for _ in range(2 * b.team_size[a])]
+    assert list(ln) == [51 + 29 * 2 * t for t in teams]
+    g = np.random.default_rng(3)
+    for step in range(4):
+        d = torch.tensor(g.integers(0, 90, b.num_cars).astype(np.int32), device="cuda")
+        b.StepEnv(d.data_ptr())
+        obs = b.CopyBuiltObs()
+        recs = b.GetAllCarStates()
+        balls = b.GetAllBallStates()
+        allp = b.GetAllBoostPadStates()
+        sz = C.sizeof(E.BoostPadState)
+        raw = bytes(allp)
+        o = 0
+        for a in range(n):
+            k = 2 * b.team_size[a]
+            cars = [recs[o + i] for i in range(k)]
+            pp = R.PADS.from_buffer_copy(raw[a * 34 * sz:(a + 1) * 34 * sz])
+            for i in range(k):
+                np.testing.assert_allclose(obs[off[o + i]:off[o + i] + ln[o + i]],
+                                           S.advanced_obs(cars, balls[a], pp, i), atol=2e-6,
+                                           err_msg=f"step {step} arena {a} car {i}")
+            o += k
+    b.BuildObs()
+    obs = b.CopyBuiltObs()
+    cars = [b.GetCarState(5, i) for i in range(6)]
+    for i in range(6):
+        c0 = sum(2 * b.team_size[x] for x in range(5))  # arena 5 is 3v3
+        np.testing.assert_allclose(obs[off[c0 + i]:off[c0 + i] + 225],
+                                   S.advanced_obs(cars, b.GetBallState(5), b.GetBoostPadStates(5), i), atol=2e-6)
+    b.ConfigureEnv(no_touch_timeout_s=0.0, auto_reset=True, obs_builder=E.OBS_DEFAULT)
+    off, ln = b.ObsLayout()
+    assert list(ln) == [51 + 19 * 2 * t for t in teams]
+    b.Destroy()
