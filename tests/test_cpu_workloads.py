@@ -155,3 +155,21 @@ def test_3v3_pileup_in_a_goal_corner(oracle, cpucheck):
     assert most >= 6, most  # car-car, car-ball and car-wall manifolds at once
     print(f"pile-up: most manifolds with contacts in a tick {most}, worst passing error {worst:.3f}, "
           f"exempt arena-ticks {len(exempt)} (first {exempt[:1]})")
+
+
+def test_extra_cases_match_oracle(oracle, cpucheck):
+    """Wheel rays on another car and on the ball, the auto-flip, and a
+    demolition through its respawn (extra_cases.py), teacher-forced against
+    the oracle with the same exemption rule; the respawn is reached."""
+    import extra_cases as X
+    lib, world = cpucheck
+    lut = WH.lut_controls()
+    cases = X.cases()
+    engs = [WH.run_cpucheck(lib, world, c, c["ticks"], lut) for c in cases]
+    failures, exempt, worst, respawned = X.check(oracle, cases, engs, lut)
+    print(f"extra cases: worst passing {worst:.3f}, exempt {[(x[0], x[1]) for x in exempt if x[3] != 'after an exempt tick']}, "
+          f"respawns {respawned}")
+    assert not failures, failures[:5]
+    assert [r[0] for r in respawned] == ["demolition respawn"]
+    auto = engs[3]
+    assert any(c[0].is_auto_flipping for c in auto.cars)

@@ -234,3 +234,24 @@ def test_full_size_runs(kind, torch_cuda):
     done = sum(x != 0 for x in r1[3])
     print(f"{kind}: {n} arenas x 16 env steps, replay bit-identical, {done} terminal arena-steps, cull {st}",
           flush=True)
+
+
+def test_extra_cases_match_oracle(oracle, torch_cuda):
+    """Wheel rays on another car and on the ball, the auto-flip and a
+    demolition through its respawn (extra_cases.py) in one GPU batch, one
+    tick per launch, teacher-forced against the oracle with the same rule."""
+    import extra_cases as X
+    cases = X.cases()
+    lut = WH.lut_controls()
+    batch = WH.engine_batch(cases)
+    try:
+        res = WH.run_engine(batch, cases, max(c["ticks"] for c in cases))
+    finally:
+        batch.Destroy()
+    failures, exempt, worst, respawned = X.check(oracle, cases, res, lut)
+    print(f"extra cases on the GPU: worst passing {worst:.3f}, "
+          f"exempt {[(x[0], x[1]) for x in exempt if x[3] != 'after an exempt tick']}, respawns {respawned}",
+          flush=True)
+    assert not failures, failures[:5]
+    assert [r[0] for r in respawned] == ["demolition respawn"]
+    assert any(c[0].is_auto_flipping for c in res[3].cars)

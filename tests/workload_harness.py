@@ -100,11 +100,19 @@ def lut_controls():
     return list(E.default_action_lut())
 
 
+def case_controls(case, t, lut):
+    """Controls of every car at tick t: the case's own "controls" (a list, or
+    a callable of the tick) when it has them, else its table actions."""
+    c = case.get("controls")
+    if c is not None:
+        return c(t) if callable(c) else c
+    return [lut[k] for k in case["actions"]]
+
+
 def run_oracle(lib, case, ticks, lut, warm=True):
     """Warm tick, reload, then `ticks` ticks under the case's actions; records
     every tick after the reload."""
     nb, no = case["blue"], case["orange"]
-    ctl = [lut[k] for k in case["actions"]]
     h = C.c_void_p(lib.ref_arena_new(nb, no))
     try:
         lib.ref_step(h, 1)
@@ -120,13 +128,13 @@ def run_oracle(lib, case, ticks, lut, warm=True):
         for i in range(nb + no):
             lib.ref_set_wheels(h, i, P.WHEEL_ZERO)
         if warm:
-            for i, c in enumerate(ctl):
+            for i, c in enumerate(case_controls(case, 0, lut)):
                 lib.ref_set_controls(h, i, C.byref(c))
             lib.ref_step(h, 1)
             load()
         res = P.Result()
         for t in range(ticks):
-            for i, c in enumerate(ctl):
+            for i, c in enumerate(case_controls(case, t, lut)):
                 lib.ref_set_controls(h, i, C.byref(c))
             lib.ref_step(h, 1)
             cars = []
@@ -151,7 +159,6 @@ def run_oracle(lib, case, ticks, lut, warm=True):
 def run_cpucheck(lib, world, case, ticks, lut, warm=True):
     """The CPU build of the engine's tick, same protocol as run_oracle."""
     nb, no = case["blue"], case["orange"]
-    ctl = [lut[k] for k in case["actions"]]
     h = C.c_void_p(lib.cc_arena_new(world, nb, no))
     try:
         def load():
@@ -165,14 +172,14 @@ def run_cpucheck(lib, world, case, ticks, lut, warm=True):
         for i in range(nb + no):
             lib.cc_set_wheels(h, i, P.WHEEL_ZERO6)
         if warm:
-            for i, c in enumerate(ctl):
+            for i, c in enumerate(case_controls(case, 0, lut)):
                 lib.cc_set_controls(h, i, C.byref(c))
             lib.cc_step(h, 1)
             load()
         res = P.Result()
         scored = 0
         for t in range(ticks):
-            for i, c in enumerate(ctl):
+            for i, c in enumerate(case_controls(case, t, lut)):
                 lib.cc_set_controls(h, i, C.byref(c))
             scored |= lib.cc_step(h, 1)
             cars = []
@@ -219,16 +226,32 @@ def load_engine(batch, cases):
 
 def run_engine(batch, cases, ticks):
     """The GPU engine (one arena per case): warm tick, reload, then `ticks`
-    launches of one tick under the cases' discrete actions, recording every
-    arena after each."""
-    idx = np.concatenate([c["actions"] for c in cases]).astype(np.int32)
+    launches of one tick, recording every arena after each.  The cars get the
+    cases' discrete actions through the device action table
+    (rsx_set_actions_from_indices), or, when a case has its own "controls",
+    every car's controls through rsx_set_all_controls."""
+    lut = lut_controls()
+    direct = any(c.get("controls") is not None for c in cases)
+
+    def set_controls(t):
+        if not direct:
+            batch.SetDiscreteActionsFromIndices(np.concatenate([c["actions"] for c in cases]).astype(np.int32))
+            return
+        arr = (E.CarControls * batch.num_cars)()
+        o = 0
+        for c in cases:
+            for i, k in enumerate(case_controls(c, t, lut)):
+                arr[o + i] = k
+            o += len(c["cars"])
+        batch.SetAllCarControls(arr)
+
     load_engine(batch, cases)
-    batch.SetDiscreteActionsFromIndices(idx)
+    set_controls(0)
     batch.Step(1)
     load_engine(batch, cases)
     res = [P.Result() for _ in cases]
     for t in range(ticks):
-        batch.SetDiscreteActionsFromIndices(idx)
+        set_controls(t)
         batch.Step(1)
         batch.Synchronize()
         cars = batch.GetAllCarStates()
@@ -290,6 +313,18 @@ def _perturbed(cars, ball, seed):
     return out, b
 
 
+RESPAWN_X = (-2304.0, -2688.0, 2304.0, 2688.0)  # RLConst.h CAR_RESPAWN_LOCATIONS_SOCCAR
+RESPAWN_Y, RESPAWN_Z = 4608.0, 36.0
+
+
+def at_respawn_spot(c):
+    """A car just respawned (Car::Respawn, Car.cpp:43-56): on one of its
+    team's four spots after one tick of falling, boost 33.33, not demolished."""
+    y = -RESPAWN_Y if c.team == 0 else RESPAWN_Y
+    return (min(abs(c.pos.x - x) for x in RESPAWN_X) < 1.0 and abs(c.pos.y - y) < 1.0 and
+            abs(c.pos.z - RESPAWN_Z) < 1.0 and abs(c.boost - 100 / 3.0) < 0.01 and not c.is_demoed)
+
+
 def oracle_teacher_forced(lib, case, eng, ticks, lut, perturb_at=-1, seed=0):
     """The oracle re-fed the engine's state every tick: tick 0 starts from the
     case (same warm protocol as run_oracle), tick t > 0 from the engine's
@@ -300,7 +335,6 @@ def oracle_teacher_forced(lib, case, eng, ticks, lut, perturb_at=-1, seed=0):
     perturb_at >= 0 moves that tick's input positions and velocities by one
     ulp (seeded), to measure how far the oracle departs from itself."""
     nb, no = case["blue"], case["orange"]
-    ctl = [lut[k] for k in case["actions"]]
     h = C.c_void_p(lib.ref_arena_new(nb, no))
     try:
         lib.ref_step(h, 1)
@@ -315,7 +349,7 @@ def oracle_teacher_forced(lib, case, eng, ticks, lut, perturb_at=-1, seed=0):
         load(case["tick"], case["cars"], case["ball"], case["pads"])
         for i in range(nb + no):
             lib.ref_set_wheels(h, i, P.WHEEL_ZERO)
-        for i, c in enumerate(ctl):
+        for i, c in enumerate(case_controls(case, 0, lut)):
             lib.ref_set_controls(h, i, C.byref(c))
         lib.ref_step(h, 1)
         res = P.Result()
@@ -327,7 +361,7 @@ def oracle_teacher_forced(lib, case, eng, ticks, lut, perturb_at=-1, seed=0):
             if t == perturb_at:
                 cars, ball = _perturbed(cars, ball, seed)
             load(case["tick"] + t, cars, ball, pads)
-            for i, c in enumerate(ctl):
+            for i, c in enumerate(case_controls(case, t, lut)):
                 lib.ref_set_controls(h, i, C.byref(c))
             lib.ref_step(h, 1)
             out = []
@@ -363,7 +397,10 @@ def teacher_forced_check(lib, cases, engs, ticks, lut, n_perturb=8):
     any code that is not bit-identical to the oracle's x86 build).  After an
     exempt tick the oracle's carried per-wheel values and broadphase cells come
     from a result the bar could not fix, so the arena's later ticks are
-    compared and reported but exempt as well.  The rest are failures."""
+    compared and reported but exempt as well.  A car respawning after a
+    demolition lands on a spot each side draws at random: at that tick its
+    position only has to be one of its team's spots on both sides.  The rest
+    are failures."""
     failures, exempt, worst = [], [], 0.0
     for case, eng in zip(cases, engs):
         ref = oracle_teacher_forced(lib, case, eng, ticks, lut)
@@ -372,8 +409,16 @@ def teacher_forced_check(lib, cases, engs, ticks, lut, n_perturb=8):
         tainted = False
         for t in range(ticks):
             f, e = errors(ref, e2, t)
+            if f and t > 0:
+                # a car respawning this tick lands on a spot each side draws
+                # at random: accepted when only its position differs and both
+                # sides put it on one of its team's spots
+                for i, (a, b) in enumerate(zip(eng.cars[t - 1], eng.cars[t])):
+                    if a.is_demoed and not b.is_demoed and all(k.startswith(f"car{i}.pos") for k in f) and \
+                            at_respawn_spot(b) and at_respawn_spot(ref.cars[t][i]):
+                        f = {}
             if not f:
-                worst = max(worst, e)
+                worst = max(worst, e) if e < 1.0 else worst
                 continue
             if tainted:
                 exempt.append((case["source"], t, round(e, 2), "after an exempt tick", sorted(f)[:4]))
