@@ -79,6 +79,8 @@ _SIGS = [
     ("rsx_get_cull_stats", _I, [_VP, C.POINTER(C.c_uint64 * 3)]),
     ("rsx_reset_cull_stats", _I, [_VP]),
     ("rsx_cull_reach", _I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    # device GAE (include/rsx_gae.h)
+    ("rsx_gae", _I, [_VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
 ]
 EXPORTED = [name for name, _, _ in _SIGS]
 

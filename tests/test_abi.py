@@ -11,7 +11,8 @@ import pytest
 import rsxlib as R
 
 ROOT = R.ROOT
-HEADERS = [os.path.join(ROOT, "include", f) for f in ("rsx.h", "rsx_sdf.h")]
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
 
 
 def declared_functions():
