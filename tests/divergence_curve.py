@@ -89,6 +89,7 @@ def oracle_rollout(lib, case, acts, ticks, lut, perturb_seed=None):
 def engine_rollout(cases, acts, ticks):
     batch = WH.engine_batch(cases)
     try:
+        WH.register_away(batch)
         WH.load_engine(batch, cases)
         batch.SetDiscreteActionsFromIndices(np.concatenate([a[0] for a in acts]))
         batch.Step(1)

@@ -88,3 +88,19 @@ def test_clip_portion_closed_form():
     r = np.array([4, 1, -6], F)
     _, _, _, portion = G.gae(r, [0, 0, 1], np.zeros(3, F), [], 0.9, 0.9, return_std=2.0, clip_range=1.0)
     assert abs(portion - (5.5 - 2.5) / 5.5) < 1e-6
+
+
+def test_argument_checks_without_device_work():
+    """rsx_gae with no steps returns at once; null buffers, negative counts and
+    null parameters are refused before any device call."""
+    import ctypes as C
+    import rsxlib as R
+    from emy_s_framework_amd.gae import GAEParams
+    lib = R.engine.load_library()
+    p = GAEParams(0.99, 0.95, 0.0, 0.0)
+    null = C.c_void_p(0)
+    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 0, C.byref(p), null) == 0
+    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 10, C.byref(p), null) != 0
+    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, -1, C.byref(p), null) != 0
+    assert lib.rsx_gae(null, null, null, null, -2, null, null, null, null, 0, C.byref(p), null) != 0
+    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 0, None, null) != 0

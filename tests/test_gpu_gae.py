@@ -88,14 +88,3 @@ def test_large_buffer_on_episode_windows(torch_cuda):
     ms = e0.elapsed_time(e1) / 5
     print(f"GAE over {T} steps: {ms:.3f} ms ({T * 17 / ms / 1e6:.1f} GB/s of inputs and outputs)", flush=True)
 
-
-def test_empty_and_invalid(torch_cuda):
-    import ctypes as C
-    from emy_s_framework_amd.gae import GAEParams
-    lib = R.engine.load_library()
-    p = GAEParams(0.99, 0.95, 0.0, 0.0)
-    null = C.c_void_p(0)
-    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 0, C.byref(p), null) == 0
-    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 10, C.byref(p), null) != 0
-    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, -1, C.byref(p), null) != 0
-    assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 0, None, null) != 0

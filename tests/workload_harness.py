@@ -224,12 +224,34 @@ def load_engine(batch, cases):
     assert o == batch.num_cars and n == batch.num_arenas
 
 
+def register_away(batch):
+    """One tick with every car high in the air, away from the field's centre.
+
+    The oracle's arena ticks once before a case is loaded (run_oracle,
+    oracle_teacher_forced) with its cars at their respawn spots (Arena::AddCar
+    respawns each car, Arena.cpp:49-61), so during the warm tick every car is
+    still registered in the broadphase there, while a new engine batch has
+    them registered where Bullet adds them, at the origin: a car resting on a
+    car near the origin would then have its warm-tick wheel rays see the other
+    car on one side only, and carry different friction factors into tick 0.
+    Cars in the air keep the per-wheel friction factors of a new batch (zero,
+    as the oracle's zeroed wheels) and reset the suspension pushback."""
+    cars = batch.GetAllCarStates()
+    for k in range(batch.num_cars):
+        # cars of one arena are consecutive, at most 6: distinct spots
+        cars[k] = P.default_car(pos=(-2500 + 1000 * (k % 6), 3000, 1000), on_ground=False)
+    batch.SetAllCarStates(cars)
+    batch.SetAllCarControls((E.CarControls * batch.num_cars)())
+    batch.Step(1)
+
+
 def run_engine(batch, cases, ticks):
-    """The GPU engine (one arena per case): warm tick, reload, then `ticks`
-    launches of one tick, recording every arena after each.  The cars get the
-    cases' discrete actions through the device action table
-    (rsx_set_actions_from_indices), or, when a case has its own "controls",
-    every car's controls through rsx_set_all_controls."""
+    """The GPU engine (one arena per case): a tick with the cars away
+    (register_away), load, warm tick, reload, then `ticks` launches of one
+    tick, recording every arena after each.  The cars get the cases' discrete
+    actions through the device action table (rsx_set_actions_from_indices),
+    or, when a case has its own "controls", every car's controls through
+    rsx_set_all_controls."""
     lut = lut_controls()
     direct = any(c.get("controls") is not None for c in cases)
 
@@ -245,6 +267,7 @@ def run_engine(batch, cases, ticks):
             o += len(c["cars"])
         batch.SetAllCarControls(arr)
 
+    register_away(batch)
     load_engine(batch, cases)
     set_controls(0)
     batch.Step(1)
