@@ -81,6 +81,7 @@ _SIGS = [
     ("rsx_cull_reach", _I, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     # device GAE (include/rsx_gae.h)
     ("rsx_gae", _I, [_VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+    ("rsx_gae_noalloc", _I, [_VP, _VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
 ]
 EXPORTED = [name for name, _, _ in _SIGS]
 

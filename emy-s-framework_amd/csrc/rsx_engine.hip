@@ -348,6 +348,12 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     if (e->device < 0 || e->device >= nd) return fail(RSX_INVALID_ARG, "device ordinal out of range");
     HIP_TRY(hipSetDevice(e->device));
     {
+        // the GAE entry point that allocates nothing finds its scratch here,
+        // before a caller can capture it in a graph
+        const rsx_status gs = gae_noalloc_setup(e->device);
+        if (gs != RSX_OK) return gs;
+    }
+    {
         int a = 0, b = 0;
         HIP_TRY(hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
         if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, e->device) != hipSuccess) b = 0;

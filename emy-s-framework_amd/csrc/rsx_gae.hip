@@ -237,7 +237,43 @@ struct GaeScratch {
 };
 GaeScratch g_scratch;
 
+constexpr int kNoAllocBlocks = (RSX_GAE_NOALLOC_MAX_STEPS + kBlock - 1) / kBlock;
+constexpr int kMaxDevices = 64;
+
 }  // namespace
+
+// Scratch of rsx_gae_noalloc: the layout above for RSX_GAE_NOALLOC_MAX_STEPS
+// steps, one copy per GPU for the life of the process.  In host code this
+// name is only a stand-in for the symbol: kernels are given the device
+// address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) int scratch[kScratchHead + 4 * kNoAllocBlocks];
+
+namespace {
+
+// Address of `scratch` on each device, looked up once per device.
+std::mutex g_noalloc_m;
+int* g_noalloc_ptr[kMaxDevices];
+
+}  // namespace
+
+namespace rsx {
+
+// Looks up the address of rsx_gae_noalloc's scratch on device `dev`, which
+// must be the current device; later calls return at once.
+int32_t gae_noalloc_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return record_failure(RSX_INVALID_ARG, "device ordinal beyond the no-alloc GAE's device table");
+    std::lock_guard<std::mutex> g(g_noalloc_m);
+    if (g_noalloc_ptr[dev]) return RSX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return record_failure(RSX_HIP_ERROR, hipGetErrorString(e));
+    if (!p) return record_failure(RSX_HIP_ERROR, "no device address for the no-alloc GAE scratch");
+    g_noalloc_ptr[dev] = (int*)p;
+    return RSX_OK;
+}
+
+}  // namespace rsx
 
 extern "C" {
 
@@ -271,6 +307,30 @@ rsx_status rsx_gae(const float* d_rews, const int8_t* d_terminals, const float* 
         }
         buf = g_scratch.buf[dev];
     }
+    return gae_launch(d_rews, d_terminals, d_val_preds, d_trunc_val_preds, num_truncs, d_advantages, d_target_values,
+                      d_returns, d_rew_clip_portion, T, params, buf, (hipStream_t)stream);
+}
+
+rsx_status rsx_gae_noalloc(const float* d_rews, const int8_t* d_terminals, const float* d_val_preds,
+                           const float* d_trunc_val_preds, int32_t num_truncs, float* d_advantages,
+                           float* d_target_values, float* d_returns, float* d_rew_clip_portion, int32_t T,
+                           const rsx_gae_params* params, void* stream) {
+    if (T > RSX_GAE_NOALLOC_MAX_STEPS)
+        return rsx::record_failure(RSX_INVALID_ARG, "T exceeds RSX_GAE_NOALLOC_MAX_STEPS (use rsx_gae)");
+    rsx_status s = gae_check_args(d_rews, d_terminals, d_val_preds, d_trunc_val_preds, num_truncs, d_advantages,
+                                  d_target_values, d_returns, T, params);
+    if (s != RSX_OK || T == 0) return s;
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return rsx::record_failure(RSX_HIP_ERROR, hipGetErrorString(e));
+    if ((s = rsx::gae_noalloc_setup(dev)) != RSX_OK) return s;
+    int* buf;
+    {
+        std::lock_guard<std::mutex> g(g_noalloc_m);
+        buf = g_noalloc_ptr[dev];
+    }
+    // every scratch word the kernels read is written earlier in the same
+    // call (block counts, their scan, the block sums): nothing to clear
     return gae_launch(d_rews, d_terminals, d_val_preds, d_trunc_val_preds, num_truncs, d_advantages, d_target_values,
                       d_returns, d_rew_clip_portion, T, params, buf, (hipStream_t)stream);
 }

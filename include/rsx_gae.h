@@ -47,6 +47,27 @@ rsx_status rsx_gae(const float* d_rews, const int8_t* d_terminals, const float* 
                    float* d_returns, float* d_rew_clip_portion, int32_t T, const rsx_gae_params* params,
                    void* stream);
 
+/* Largest T that rsx_gae_noalloc takes (its fixed scratch: 16 bytes per
+ * 1024 steps, 1 MiB per GPU). */
+#define RSX_GAE_NOALLOC_MAX_STEPS (1 << 26)
+
+/* No counterpart in the reference: rsx_gae for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as rsx_gae, but no
+ * hipMalloc or hipFree and no device synchronization: the scratch is a fixed
+ * array in the library's device memory, one copy per GPU held until the
+ * process ends, whose address is looked up once per device (by rsx_create
+ * or by the first call on that device; make one call before a capture on a
+ * device without an engine).  T above RSX_GAE_NOALLOC_MAX_STEPS is refused
+ * with RSX_INVALID_ARG before any device work.  Every call on one GPU shares
+ * that GPU's array and each call writes all of the array it reads, so calls
+ * in order on one stream are safe; calls that overlap on one GPU (other
+ * streams or threads) race. */
+rsx_status rsx_gae_noalloc(const float* d_rews, const int8_t* d_terminals, const float* d_val_preds,
+                           const float* d_trunc_val_preds, int32_t num_truncs, float* d_advantages,
+                           float* d_target_values, float* d_returns, float* d_rew_clip_portion, int32_t T,
+                           const rsx_gae_params* params, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,3 +104,24 @@ def test_argument_checks_without_device_work():
     assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, -1, C.byref(p), null) != 0
     assert lib.rsx_gae(null, null, null, null, -2, null, null, null, null, 0, C.byref(p), null) != 0
     assert lib.rsx_gae(null, null, null, null, 0, null, null, null, null, 0, None, null) != 0
+
+
+def test_noalloc_refuses_buffers_past_its_scratch():
+    """rsx_gae_noalloc refuses T above RSX_GAE_NOALLOC_MAX_STEPS (read from the
+    header) before any other check or device work; at the maximum the same
+    call gets as far as the null-buffer check."""
+    import ctypes as C
+    import re
+    import rsxlib as R
+    from emy_s_framework_amd.gae import GAEParams
+    hdr = open(os.path.join(ROOT, "include", "rsx_gae.h")).read()
+    mx = eval(re.search(r"#define RSX_GAE_NOALLOC_MAX_STEPS \(([^)]*)\)", hdr).group(1))
+    assert mx == 1 << 26
+    lib = R.engine.load_library()
+    p = GAEParams(0.99, 0.95, 0.0, 0.0)
+    null = C.c_void_p(0)
+    assert lib.rsx_gae_noalloc(null, null, null, null, 0, null, null, null, null, mx + 1, C.byref(p), null) != 0
+    assert b"RSX_GAE_NOALLOC_MAX_STEPS" in lib.rsx_last_error()
+    assert lib.rsx_gae_noalloc(null, null, null, null, 0, null, null, null, null, mx, C.byref(p), null) != 0
+    assert b"null device buffer" in lib.rsx_last_error()
+    assert lib.rsx_gae_noalloc(null, null, null, null, 0, null, null, null, null, 0, C.byref(p), null) == 0
