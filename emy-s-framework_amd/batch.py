@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, EnvConfig, NUM_PADS,
-                      PadStates, CARS_BLUE_FIRST, Tuning, TuneResult)
+                      PadStates, CARS_BLUE_FIRST, REWARD, MAX_TRAINING_REWARDS, TrainingRewardEntry, Tuning,
+                      TuneResult)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
@@ -45,6 +46,11 @@ _SIGS = [
     ("rsx_reset_arena", _I, [_VP, _I]),
     ("rsx_reset_all_arenas", _I, [_VP, _VP]),
     ("rsx_set_all_controls", _I, [_VP, C.POINTER(CarControls), _I]),
+    ("rsx_set_car_controls", _I, [_VP, _I, _I, C.POINTER(CarControls)]),
+    ("rsx_configure_training_rewards", _I, [_VP, C.POINTER(TrainingRewardEntry), _I]),
+    ("rsx_get_training_rewards", _I, [_VP, C.POINTER(TrainingRewardEntry), _I, C.POINTER(C.c_int32)]),
+    ("rsx_set_reset_curriculum", _I, [_VP, C.c_float, C.c_float, C.c_float, C.c_float]),
+    ("rsx_get_reset_curriculum", _I, [_VP, C.POINTER(C.c_float)]),
     ("rsx_configure_env", _I, [_VP, C.POINTER(EnvConfig)]),
     ("rsx_get_env_config", _I, [_VP, C.POINTER(EnvConfig)]),
     ("rsx_step_env", _I, [_VP, _VP, _VP]),
@@ -256,12 +262,52 @@ class RocketSimBatch:
 
     # --- env half of the step -------------------------------------------------
     def ConfigureEnv(self, goal=1.0, concede=-1.0, touch=1.0, vel_ball_to_goal=1.0, vel_player_to_ball=1.0,
-                     no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True, obs_builder=0):
-        """ConfigureTrainingRewards + ConfigureTrainingTerminals; obs_builder
-        0 (DefaultObs) or 1 (AdvancedObs, BuildAdvancedObsAndDefaultMasks)."""
+                     no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True, obs_builder=0,
+                     no_touch_stagger=False):
+        """ConfigureTrainingTerminals and the stub rewards' weights;
+        obs_builder 0 (DefaultObs) or 1 (AdvancedObs,
+        BuildAdvancedObsAndDefaultMasks); no_touch_stagger: resets start the
+        no-touch timer at a random age, as the reference's ResetArena."""
         c = EnvConfig(goal, concede, touch, vel_ball_to_goal, vel_player_to_ball, no_touch_timeout_s,
-                      int(goal_terminal), int(auto_reset), int(obs_builder))
+                      int(goal_terminal), int(auto_reset), int(obs_builder), int(no_touch_stagger))
         _check(self._lib.rsx_configure_env(self._h, C.byref(c)))
+
+    def ConfigureTrainingRewards(self, entries):
+        """ConfigureTrainingRewards (RocketSimCuda.h:329).  entries: a list of
+        TrainingRewardEntry, or of dicts with "id" (an RSX_REWARD_* number or
+        a TrainingRewardID name), "weight" and optional "params" (up to 10),
+        "zero_sum", "team_spirit", "opponent_scale" (default 1).  An empty
+        list returns to the stub rewards."""
+        if len(entries) > MAX_TRAINING_REWARDS:
+            raise ValueError("at most 16 reward entries")
+        arr = (TrainingRewardEntry * max(1, len(entries)))()
+        for k, e in enumerate(entries):
+            if isinstance(e, TrainingRewardEntry):
+                arr[k] = e
+                continue
+            rid = REWARD[e["id"]] if isinstance(e["id"], str) else int(e["id"])
+            params = list(e.get("params", ())) + [0.0] * 10
+            arr[k] = TrainingRewardEntry(rid, float(e.get("weight", 1.0)), (C.c_float * 10)(*params[:10]),
+                                         int(bool(e.get("zero_sum", False))), float(e.get("team_spirit", 0.0)),
+                                         float(e.get("opponent_scale", 1.0)))
+        _check(self._lib.rsx_configure_training_rewards(self._h, arr, len(entries)))
+
+    def GetTrainingRewards(self):
+        n = C.c_int32(0)
+        _check(self._lib.rsx_get_training_rewards(self._h, None, 0, C.byref(n)))
+        arr = (TrainingRewardEntry * max(1, n.value))()
+        _check(self._lib.rsx_get_training_rewards(self._h, arr, n.value, C.byref(n)))
+        return [arr[k] for k in range(n.value)]
+
+    def SetResetCurriculum(self, kickoff_w, fuzzed_w, aerial_w, chase_w=0.0):
+        """SetResetCurriculum (RocketSimCuda.h:283): relative weights of the
+        kickoff, fuzzed kickoff, aerial and ball-chase resets."""
+        _check(self._lib.rsx_set_reset_curriculum(self._h, kickoff_w, fuzzed_w, aerial_w, chase_w))
+
+    def GetResetCurriculum(self):
+        w = (C.c_float * 4)()
+        _check(self._lib.rsx_get_reset_curriculum(self._h, w))
+        return list(w)
 
     def StepEnv(self, device_actions_ptr, stream=None):
         """Fused env step from a device int32 array of num_cars action indices."""
@@ -302,6 +348,10 @@ class RocketSimBatch:
         out = np.zeros(self.num_arenas, dtype=np.uint8)
         _check(self._lib.rsx_copy_terminals(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), self.num_arenas))
         return out
+
+    def SetCarControls(self, arena, car, controls):
+        """SetCarControls (RocketSimCuda.h:288)."""
+        _check(self._lib.rsx_set_car_controls(self._h, arena, car, C.byref(controls)))
 
     def SetAllCarControls(self, controls):
         """controls: ctypes array of CarControls, arena-major, 2*T per arena."""

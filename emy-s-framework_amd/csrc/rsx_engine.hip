@@ -98,6 +98,10 @@ struct rsx_engine {
     uint8_t* d_mask_tables = nullptr;
     uint8_t* d_masks = nullptr;
     bool masks_on = false;
+    // training reward table (rsx_configure_training_rewards); 0 entries: stub rewards
+    RewardEntry* d_rt = nullptr;
+    int rt_n = 0;
+    std::vector<rsx_training_reward_entry> rt_host;
     // captured env steps (rsx_capture_env_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -115,30 +119,33 @@ void engine_set_cull(rsx_engine* e, const sdf::Cull& c) {
 
 namespace {
 
-// Kickoff reset of one arena column in place: keeps car ids, teams and the
-// carried wheel values (Car::SetState does not touch them), advances the
-// arena's RNG stream, bumps the episode counter.
+// Reset of one arena column in place under the env's reset curriculum (the
+// plain kickoff reset when E is null): keeps car ids, teams and the carried
+// wheel values (Car::SetState does not touch them), advances the arena's RNG
+// stream, bumps the episode counter.
 template <int MAXC>
-__host__ __device__ void reset_column(const Col& col, const CarGeom& cg) {
+__host__ __device__ void reset_column(const Col& col, const CarGeom& cg, const EnvConfig* E) {
     using L = Layout<MAXC>;
     Arena<MAXC> A;
     unpack_arena<MAXC>(col, A, cg);
-    arena_reset_kickoff<MAXC>(A, cg);
+    if (E) arena_reset_curriculum<MAXC>(A, cg, E->reset_w);
+    else arena_reset_kickoff<MAXC>(A, cg);
     A.tick = 0;
+    const float no_touch = E ? reset_no_touch(A.rng, *E) : 0.f;
     pack_arena<MAXC>(col, A);
     col.set(L::kArenaBase + kArFlags, 0u);
-    col.setf(L::kArenaBase + kArNoTouch, 0.f);
+    col.setf(L::kArenaBase + kArNoTouch, no_touch);
     col.set(L::kArenaBase + kArEpisode, col.get(L::kArenaBase + kArEpisode) + 1u);
 }
 
-// Kickoff reset of the arenas whose mask byte is non-zero (all when mask is null).
+// Reset of the arenas whose mask byte is non-zero (all when mask is null).
 template <int MAXC>
-__global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint8_t* mask, CarGeom cg) {
+__global__ void __launch_bounds__(64) k_reset(uint32_t* words, int n, const uint8_t* mask, CarGeom cg, EnvConfig E) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     if (mask && !mask[a]) return;
     Col col{words + a, n};
-    reset_column<MAXC>(col, cg);
+    reset_column<MAXC>(col, cg, &E);
 }
 
 // DefaultObs rows of the current state with zero previous actions (the
@@ -228,7 +235,7 @@ void host_reset_arena(std::vector<uint32_t>& colbuf, const CarGeom& g, const Bal
     Col c{colbuf.data(), 1};
     pack_arena<MAXC>(c, A);
     c.set(Layout<MAXC>::kArenaBase + kArNcars, (uint32_t)A.ncars);
-    reset_column<MAXC>(c, g);
+    reset_column<MAXC>(c, g, nullptr);
     c.set(Layout<MAXC>::kArenaBase + kArEpisode, 0u);
 }
 
@@ -279,6 +286,8 @@ hipError_t env_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, h
     v.term = e->d_term;
     v.masks = e->masks_on ? e->d_masks : nullptr;
     v.mask_tables = e->d_mask_tables;
+    v.rt = e->d_rt;
+    v.rt_n = e->rt_n;
     int nonempty = 0;
     for (const Bucket& b : e->bucket) nonempty += b.n > 0;
     if (nonempty > 1) {
@@ -422,6 +431,10 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         E.goal_terminal = 1;
         E.auto_reset = 1;
         E.obs_kind = kObsDefault;
+        E.reset_w[0] = 1.f;
+        E.reset_w[1] = E.reset_w[2] = E.reset_w[3] = 0.f;
+        E.no_touch_stagger = 0;
+        HIP_TRY(hipMalloc((void**)&e->d_rt, sizeof(RewardEntry) * kMaxRewardEntries));
         if (!obs_pad_map(e->hw.pads, E.pad_map)) return fail(RSX_MESH_LOAD_FAILED, "boost pad layout does not match");
         // the obs buffer holds the longer (AdvancedObs) rows, so its address
         // survives a change of observation builder
@@ -492,6 +505,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
     (void)hipFree(e->d_mask_tables);
     (void)hipFree(e->d_masks);
     (void)hipFree(e->d_idx_stage);
+    (void)hipFree(e->d_rt);
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris, e->dw.cell_ent,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
     for (void* p : ptrs) (void)hipFree(p);
@@ -639,9 +653,9 @@ rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena) {
     rsx_status s = column_get(e, arena, col, b, local);
     if (s != RSX_OK) return s;
     Col c{col.data(), 1};
-    if (b->maxc == 2) reset_column<2>(c, e->hw.cg);
-    else if (b->maxc == 4) reset_column<4>(c, e->hw.cg);
-    else reset_column<6>(c, e->hw.cg);
+    if (b->maxc == 2) reset_column<2>(c, e->hw.cg, &e->env);
+    else if (b->maxc == 4) reset_column<4>(c, e->hw.cg, &e->env);
+    else reset_column<6>(c, e->hw.cg, &e->env);
     return column_put(e, b, local, col);
 }
 
@@ -653,9 +667,11 @@ rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream) {
         Bucket& b = e->bucket[k];
         if (!b.n) continue;
         dim3 grid((b.n + 63) / 64), block(64);
-        if (b.maxc == 2) hipLaunchKernelGGL(k_reset<2>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
-        else if (b.maxc == 4) hipLaunchKernelGGL(k_reset<4>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
-        else hipLaunchKernelGGL(k_reset<6>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg);
+        if (b.maxc == 2)
+            hipLaunchKernelGGL(k_reset<2>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg, e->env);
+        else if (b.maxc == 4)
+            hipLaunchKernelGGL(k_reset<4>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg, e->env);
+        else hipLaunchKernelGGL(k_reset<6>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg, e->env);
         HIP_TRY(hipGetLastError());
     }
     return RSX_OK;
@@ -691,6 +707,18 @@ rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int
         if (e->bucket[k].n)
             HIP_TRY(hipMemcpy(e->bucket[k].d_ctrl, per[k].data(), per[k].size() * sizeof(rsx_controls),
                               hipMemcpyHostToDevice));
+    return RSX_OK;
+}
+
+rsx_status rsx_set_car_controls(rsx_engine* e, int32_t arena, int32_t car, const rsx_controls* c) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!c) return fail(RSX_INVALID_ARG, "null controls");
+    if (arena < 0 || arena >= e->num_arenas) return fail(RSX_INVALID_ARG, "arena index out of range");
+    if (car < 0 || car >= 2 * e->arena_team[arena]) return fail(RSX_INVALID_ARG, "car index out of range");
+    const Bucket& b = e->bucket[e->arena_bucket[arena]];
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpy(b.d_ctrl + (size_t)e->arena_local[arena] * b.maxc + car, c, sizeof(rsx_controls),
+                      hipMemcpyHostToDevice));
     return RSX_OK;
 }
 
@@ -907,6 +935,7 @@ rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c) {
     E.no_touch_timeout = c->no_touch_timeout_s;
     E.goal_terminal = c->goal_terminal != 0;
     E.auto_reset = c->auto_reset != 0;
+    E.no_touch_stagger = c->no_touch_stagger != 0;
     if (c->obs_builder != RSX_OBS_DEFAULT && c->obs_builder != RSX_OBS_ADVANCED)
         return fail(RSX_INVALID_ARG, "obs_builder must be RSX_OBS_DEFAULT or RSX_OBS_ADVANCED");
     if (c->obs_builder != E.obs_kind) {
@@ -938,6 +967,60 @@ rsx_status rsx_get_env_config(const rsx_engine* e, rsx_env_config* c) {
     c->goal_terminal = E.goal_terminal;
     c->auto_reset = E.auto_reset;
     c->obs_builder = E.obs_kind;
+    c->no_touch_stagger = E.no_touch_stagger;
+    return RSX_OK;
+}
+
+rsx_status rsx_configure_training_rewards(rsx_engine* e, const rsx_training_reward_entry* entries, int32_t count) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (count < 0 || count > RSX_MAX_TRAINING_REWARDS)
+        return fail(RSX_INVALID_ARG, "reward table must hold 0..RSX_MAX_TRAINING_REWARDS entries");
+    if (count > 0 && !entries) return fail(RSX_INVALID_ARG, "null reward table");
+    RewardEntry t[kMaxRewardEntries];
+    for (int k = 0; k < count; k++) {
+        const rsx_training_reward_entry& s = entries[k];
+        if (s.id >= kRwCount) return fail(RSX_INVALID_ARG, "unknown reward id " + std::to_string((int)s.id));
+        t[k].id = s.id;
+        t[k].weight = s.weight;
+        for (int p = 0; p < kRewardParams; p++) t[k].params[p] = s.params[p];
+        t[k].zero_sum = s.is_zero_sum != 0;
+        t[k].team_spirit = s.team_spirit;
+        t[k].opp_scale = s.opponent_scale;
+    }
+    HIP_TRY(hipSetDevice(e->device));
+    if (count > 0) HIP_TRY(hipMemcpy(e->d_rt, t, sizeof(RewardEntry) * count, hipMemcpyHostToDevice));
+    e->rt_n = count;
+    e->rt_host.assign(entries, entries + count);
+    e->gen++;
+    return RSX_OK;
+}
+
+rsx_status rsx_get_training_rewards(const rsx_engine* e, rsx_training_reward_entry* out, int32_t capacity,
+                                    int32_t* count) {
+    if (!e) return fail(RSX_INVALID_ARG, "engine is null");
+    if (count) *count = e->rt_n;
+    if (!out) return RSX_OK;
+    if (capacity < e->rt_n) return fail(RSX_INVALID_ARG, "capacity below the table size");
+    for (int k = 0; k < e->rt_n; k++) out[k] = e->rt_host[k];
+    return RSX_OK;
+}
+
+rsx_status rsx_set_reset_curriculum(rsx_engine* e, float kickoff_w, float fuzzed_w, float aerial_w, float chase_w) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    auto clean = [](float w) { return (!(w >= 0.f) || w > 1e6f) ? 0.f : w; };
+    float* w = e->env.reset_w;
+    w[0] = clean(kickoff_w);
+    w[1] = clean(fuzzed_w);
+    w[2] = clean(aerial_w);
+    w[3] = clean(chase_w);
+    if (w[0] + w[1] + w[2] + w[3] <= 0.f) w[0] = 1.f;
+    e->gen++;
+    return RSX_OK;
+}
+
+rsx_status rsx_get_reset_curriculum(const rsx_engine* e, float* w4) {
+    if (!e || !w4) return fail(RSX_INVALID_ARG, "null argument");
+    for (int k = 0; k < 4; k++) w4[k] = e->env.reset_w[k];
     return RSX_OK;
 }
 

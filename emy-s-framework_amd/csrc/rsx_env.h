@@ -46,6 +46,8 @@ struct EnvConfig {
     int auto_reset;           // kickoff reset of terminal arenas inside the step
     int obs_kind;             // kObsDefault or kObsAdvanced
     int8_t pad_map[kNumPads]; // obs pad slot i -> engine pad index (GameState boostPadIndexMap)
+    float reset_w[4];         // reset curriculum weights: kickoff, fuzzed kickoff, aerial, ball chase
+    int no_touch_stagger;     // a reset starts the no-touch timer at U(0, 0.9) x the timeout
 };
 
 struct StepOut {

@@ -24,7 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/rsx_types.h"
-#include "rsx_env.h"
+#include "rsx_train.h"
 #include "rsx_sdf.h"
 
 namespace rsx {
@@ -67,6 +67,8 @@ struct EnvArgs {
     uint8_t* term;
     uint8_t* masks;               // DefaultAction mask rows (kNumDefaultActions bytes per car), or null
     const uint8_t* mask_tables;   // ground, air, boost, jump masks of the default table
+    const RewardEntry* rt;        // training reward table (device), used when rt_n > 0
+    int rt_n;                     // 0: the four stub rewards of EnvConfig
 };
 
 struct LaunchShape {

@@ -108,6 +108,9 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
         prev[i][6] = c.boost ? 1.f : 0.f;
         prev[i][7] = c.handbrake ? 1.f : 0.f;
     }
+    // the state the training rewards compare against (SnapshotTrainingState)
+    PrevSnap<MAXC> P;
+    if (V.rt_n > 0) snapshot_prev<MAXC>(A, P);
     TickCtx T;
     tick_ctx(T, G, lds_wave, la.own, kPoolOffset);
     TickScratch<MAXC> W;
@@ -132,13 +135,22 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     }
     float no_touch = col.getf(L::kArenaBase + kArNoTouch);
     const uint8_t tt = step_terminal(goal, any_touch, (float)V.ticks * (1.f / kTickRate), no_touch, E);
-    for (int i = 0; i < A.ncars; i++) V.rew[c0 + i] = stub_reward<MAXC>(A, i, goal, touched[i], E);
+    if (V.rt_n > 0) {
+        // the configured training rewards (BuildRewardsAndTerminals); the
+        // goal term reads the episode's goal flag and scoring team
+        float r[MAXC];
+        const int goal_team = (flags & kFlagGoal) ? ((flags & kFlagGoalOrange) ? 1 : 0) : -1;
+        train_rewards<MAXC>(A, P, V.rt, V.rt_n, goal_team, A.tick, V.ticks, r);
+        for (int i = 0; i < A.ncars; i++) V.rew[c0 + i] = r[i];
+    } else {
+        for (int i = 0; i < A.ncars; i++) V.rew[c0 + i] = stub_reward<MAXC>(A, i, goal, touched[i], E);
+    }
     uint32_t episode = col.get(L::kArenaBase + kArEpisode);
     if (tt != kNotTerminal && E.auto_reset) {
-        arena_reset_kickoff<MAXC>(A, G.cg);
+        arena_reset_curriculum<MAXC>(A, G.cg, E.reset_w);
         A.tick = 0;
         flags &= kFlagContactOverflow;
-        no_touch = 0.f;
+        no_touch = reset_no_touch(A.rng, E);
         episode++;
         for (int i = 0; i < A.ncars; i++)
             for (int k = 0; k < kActionFloats; k++) prev[i][k] = 0.f;

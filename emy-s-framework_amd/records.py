@@ -79,7 +79,23 @@ class EnvConfig(C.Structure):
     _fields_ = [("goal_weight", C.c_float), ("concede_scale", C.c_float), ("touch_weight", C.c_float),
                 ("vel_ball_to_goal_weight", C.c_float), ("vel_player_to_ball_weight", C.c_float),
                 ("no_touch_timeout_s", C.c_float), ("goal_terminal", C.c_int32), ("auto_reset", C.c_int32),
-                ("obs_builder", C.c_int32)]
+                ("obs_builder", C.c_int32), ("no_touch_stagger", C.c_int32)]
+
+
+class TrainingRewardEntry(C.Structure):
+    """rsx_training_reward_entry = TrainingRewardEntry (RocketSimCuda.h:235-242)."""
+    _fields_ = [("id", C.c_uint8), ("weight", C.c_float), ("params", C.c_float * 10), ("is_zero_sum", C.c_uint8),
+                ("team_spirit", C.c_float), ("opponent_scale", C.c_float)]
+
+
+# TrainingRewardID (RocketSimCuda.h:200-231), the RSX_REWARD_* values
+REWARD_IDS = ["UNKNOWN", "GOAL_REWARD", "VELOCITY_BALL_TO_GOAL", "VELOCITY_PLAYER_TO_BALL", "FACE_BALL", "TOUCH_BALL",
+              "TOUCH_ACCEL", "STRONG_TOUCH", "SPEED", "VELOCITY", "AIR", "WAVEDASH", "PICKUP_BOOST", "SAVE_BOOST",
+              "BUMP", "BUMPED_PENALTY", "DEMO", "DEMOED_PENALTY", "KICKOFF_PROXIMITY", "TEAMMATE_BUMP_PENALTY",
+              "PASS_INCENTIVE", "TEAM_PASSING", "MIXED_INITIATE", "MIXED_DOUBLETAP", "SPEED_AERIAL",
+              "PRESSURE_FLICK", "MAWKZY_FLICK", "AIR_CARRY", "ANTI_BALL_STACK"]
+REWARD = {n: i for i, n in enumerate(REWARD_IDS)}
+MAX_TRAINING_REWARDS = 16
 
 
 OBS_DEFAULT = 0
@@ -91,6 +107,7 @@ CARS_ALTERNATING = 1
 
 
 assert C.sizeof(CarControls) == 24
+assert C.sizeof(TrainingRewardEntry) == 60
 assert C.sizeof(CarState) == 248
 assert C.sizeof(ArenaInfo) == 32
 

@@ -94,16 +94,20 @@ rsx_status rsx_get_all_pad_states(rsx_engine* e, rsx_pad_state* out, int32_t cou
 rsx_status rsx_set_all_pad_states(rsx_engine* e, const rsx_pad_state* in, int32_t count);
 rsx_status rsx_get_all_arena_infos(rsx_engine* e, rsx_arena_info* out, int32_t count);
 
-/* Kickoff reset of one arena / of every arena (RocketSimCuda.h:275, 278;
- * RocketSimCudaApi.cuh:203-330 with the pure-kickoff curriculum): tick count 0,
- * goal flag cleared, cars at a random kickoff permutation drawn from the
- * arena's own RNG stream, ball at rest at centre, every pad active. */
+/* Reset of one arena / of every arena (RocketSimCuda.h:275, 278;
+ * RocketSimCudaApi.cuh:203-330): with the default (kickoff-only) curriculum,
+ * tick count 0, goal flag cleared, cars at a random kickoff permutation drawn
+ * from the arena's own RNG stream, ball at rest at centre, every pad active;
+ * otherwise the reset rsx_set_reset_curriculum draws. */
 rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena);
 rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream);
 
 /* Controls of every car (host array, rsx_num_cars records); replaces
  * SetAllCarControls (RocketSimCuda.h:291). */
 rsx_status rsx_set_all_controls(rsx_engine* e, const rsx_controls* controls, int32_t count);
+/* SetCarControls (RocketSimCuda.h:288): one car's controls (car index within
+ * its arena). */
+rsx_status rsx_set_car_controls(rsx_engine* e, int32_t arena, int32_t car, const rsx_controls* controls);
 
 /* Discrete actions.  rsx_upload_action_lut replaces UploadDiscreteActionLut
  * (RocketSimCuda.h:294); rsx_set_actions_from_indices replaces
@@ -134,6 +138,9 @@ typedef struct rsx_env_config {
     int32_t goal_terminal;             /* GoalScoreCondition on; default 1 */
     int32_t auto_reset;                /* kickoff reset of terminal arenas inside rsx_step_env; default 1 */
     int32_t obs_builder;               /* RSX_OBS_DEFAULT (default) or RSX_OBS_ADVANCED */
+    int32_t no_touch_stagger;          /* a reset starts the no-touch timer at a uniform draw in
+                                          [0, 0.9) x the timeout (ResetArena, RocketSimCudaApi.cuh:396-406);
+                                          default 0: at 0, as NoTouchCondition::Reset */
 } rsx_env_config;
 
 /* Observation builders.  RSX_OBS_DEFAULT: RLGym DefaultObs (DefaultObs.cpp:
@@ -146,9 +153,79 @@ typedef struct rsx_env_config {
 #define RSX_OBS_DEFAULT 0
 #define RSX_OBS_ADVANCED 1
 
-/* ConfigureTrainingRewards + ConfigureTrainingTerminals (RocketSimCuda.h:329-330). */
+/* ConfigureTrainingTerminals (RocketSimCuda.h:330) and the stub rewards. */
 rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c);
 rsx_status rsx_get_env_config(const rsx_engine* e, rsx_env_config* c);
+
+/* Training rewards of the reference's GPU path: TrainingRewardID
+ * (RocketSimCuda.h:200-231), evaluated as training_eval_reward
+ * (RocketSimCudaRewards.cuh:36-287) on the state after the env step and the
+ * snapshot before it (SnapshotTrainingState, RocketSimCuda.h:333). */
+#define RSX_REWARD_UNKNOWN 0
+#define RSX_REWARD_GOAL 1                   /* params[0]: the scale of a conceded goal */
+#define RSX_REWARD_VELOCITY_BALL_TO_GOAL 2  /* params[0] > 0.5: towards the own goal */
+#define RSX_REWARD_VELOCITY_PLAYER_TO_BALL 3
+#define RSX_REWARD_FACE_BALL 4
+#define RSX_REWARD_TOUCH_BALL 5
+#define RSX_REWARD_TOUCH_ACCEL 6
+#define RSX_REWARD_STRONG_TOUCH 7           /* params[0]: least hit speed change, params[1]: full-reward change */
+#define RSX_REWARD_SPEED 8
+#define RSX_REWARD_VELOCITY 9               /* params[0]: scale */
+#define RSX_REWARD_AIR 10
+#define RSX_REWARD_WAVEDASH 11
+#define RSX_REWARD_PICKUP_BOOST 12
+#define RSX_REWARD_SAVE_BOOST 13            /* params[0]: exponent */
+#define RSX_REWARD_BUMP 14
+#define RSX_REWARD_BUMPED_PENALTY 15
+#define RSX_REWARD_DEMO 16
+#define RSX_REWARD_DEMOED_PENALTY 17
+#define RSX_REWARD_KICKOFF_PROXIMITY 18
+#define RSX_REWARD_TEAMMATE_BUMP_PENALTY 19
+#define RSX_REWARD_PASS_INCENTIVE 20        /* 20-24: reserved team ids, score 0 as in the reference */
+#define RSX_REWARD_TEAM_PASSING 21
+#define RSX_REWARD_MIXED_INITIATE 22
+#define RSX_REWARD_MIXED_DOUBLETAP 23
+#define RSX_REWARD_SPEED_AERIAL 24
+#define RSX_REWARD_PRESSURE_FLICK 25
+#define RSX_REWARD_MAWKZY_FLICK 26
+#define RSX_REWARD_AIR_CARRY 27
+#define RSX_REWARD_ANTI_BALL_STACK 28
+#define RSX_MAX_TRAINING_REWARDS 16
+#define RSX_TRAINING_REWARD_PARAMS 10
+
+/* TrainingRewardEntry (RocketSimCuda.h:235-242), same field order and
+ * layout.  A zero-sum entry gives each car
+ * raw * (1 - team_spirit) + team_mean * team_spirit - opponent_mean * opponent_scale. */
+typedef struct rsx_training_reward_entry {
+    uint8_t id;
+    float weight;
+    float params[RSX_TRAINING_REWARD_PARAMS];
+    uint8_t is_zero_sum;
+    float team_spirit;
+    float opponent_scale;
+} rsx_training_reward_entry;
+
+/* ConfigureTrainingRewards (RocketSimCuda.h:329): from the next env step on,
+ * each car's reward is the table's weighted sum, in table order, instead of
+ * the stub rewards of rsx_env_config (count 0 returns to those).  The goal
+ * term reads the episode's goal flag (rsx_arena_info.goal_scored / goal_team,
+ * set once per episode as in the reference).  Unknown ids: RSX_INVALID_ARG. */
+rsx_status rsx_configure_training_rewards(rsx_engine* e, const rsx_training_reward_entry* entries, int32_t count);
+rsx_status rsx_get_training_rewards(const rsx_engine* e, rsx_training_reward_entry* out, int32_t capacity,
+                                    int32_t* count);
+
+/* SetResetCurriculum (RocketSimCuda.h:283; ResetArena,
+ * RocketSimCudaApi.cuh:203-418): relative weights of the four resets every
+ * later reset (rsx_reset_arena, rsx_reset_all_arenas, the env step's
+ * auto-reset) chooses from: kickoff; fuzzed kickoff (ball within 180 x 220 uu
+ * of centre with a mild velocity, cars +-40 uu off their spots); aerial (ball
+ * at 420-780 uu height within 1400 x 1600 uu of centre, one car below and
+ * behind it in the air, moving at it); ball chase (ball within 1900 x 2200
+ * uu, every car 450-1850 uu from it on the ground, facing it).  Weights that
+ * are negative, NaN or above 1e6 count as 0; all 0 means kickoff only (the
+ * default).  The draws come from each arena's own RNG stream. */
+rsx_status rsx_set_reset_curriculum(rsx_engine* e, float kickoff_w, float fuzzed_w, float aerial_w, float chase_w);
+rsx_status rsx_get_reset_curriculum(const rsx_engine* e, float* weights4);
 
 /* One fused env step for every arena, asynchronous on `stream`: device int32
  * action indices (rsx_num_cars, clamped into the action table) -> controls,

@@ -4,7 +4,8 @@
 // Nothing in the engine's product path loads this library.
 #include "../../emy-s-framework_amd/csrc/rsx_host.h"
 #include "../../emy-s-framework_amd/csrc/rsx_soa.h"
-#include "../../emy-s-framework_amd/csrc/rsx_env.h"
+#include "../../emy-s-framework_amd/csrc/rsx_train.h"
+#include "../../include/rsx.h"
 
 #include <cstring>
 #include <string>
@@ -324,3 +325,54 @@ extern "C" int cc_tri_rank(void* w, int32_t* out, int cap) {
 // Whether any tick since the arena was created ran out of contact-manifold
 // storage (MfList::overflow).
 extern "C" int cc_contact_overflow(void* p) { return ((CcArena*)p)->W.L.overflow ? 1 : 0; }
+
+// The training rewards of one arena (rsx_train.h train_rewards) from public
+// records: n cars after the step and at its start, the ball after and at the
+// start, the reward table, the episode's scoring team (-1 none), the tick
+// count after the step and the step's ticks.  out: n rewards.
+extern "C" void cc_train_rewards(void* w, const rsx_car_state* cars, const rsx_car_state* prev, int n,
+                                 const rsx_ball_state* ball, const rsx_ball_state* prev_ball,
+                                 const rsx_training_reward_entry* entries, int ne, int goal_team, uint64_t tick,
+                                 int step_ticks, float* out) {
+    const CarGeom& g = ((CcWorld*)w)->w.cg;
+    auto load = [&](Arena<6>& A, const rsx_car_state* cs, const rsx_ball_state* b) {
+        memset(&A, 0, sizeof(A));
+        A.ncars = n;
+        for (int i = 0; i < n; i++) {
+            car_from_public(A.car[i], cs[i], g);
+            A.car[i].id = cs[i].id;
+            A.car[i].team = cs[i].team;
+        }
+        ball_from_public(A.ball, *b);
+        A.tick = tick;
+    };
+    static Arena<6> A, B;
+    load(A, cars, ball);
+    load(B, prev, prev_ball);
+    PrevSnap<6> P;
+    snapshot_prev<6>(B, P);
+    RewardEntry t[kMaxRewardEntries];
+    for (int k = 0; k < ne && k < kMaxRewardEntries; k++) {
+        t[k].id = entries[k].id;
+        t[k].weight = entries[k].weight;
+        for (int q = 0; q < kRewardParams; q++) t[k].params[q] = entries[k].params[q];
+        t[k].zero_sum = entries[k].is_zero_sum;
+        t[k].team_spirit = entries[k].team_spirit;
+        t[k].opp_scale = entries[k].opponent_scale;
+    }
+    train_rewards<6>(A, P, t, ne < kMaxRewardEntries ? ne : kMaxRewardEntries, goal_team, tick, step_ticks, out);
+}
+
+// One curriculum reset of a fresh arena (rsx_train.h arena_reset_curriculum)
+// with the given RNG seed and weights; returns the mode and writes the cars'
+// and the ball's public records.
+extern "C" int cc_curriculum_reset(void* w, int n_blue, int n_orange, uint32_t seed, const float* weights,
+                                   rsx_car_state* cars, rsx_ball_state* ball) {
+    auto* a = (CcArena*)cc_arena_new(w, n_blue, n_orange);
+    a->A.rng = seed ? seed : 1u;
+    const int mode = arena_reset_curriculum<6>(a->A, a->world->w.cg, weights);
+    for (int i = 0; i < a->A.ncars; i++) car_to_public(a->A.car[i], cars[i]);
+    ball_to_public(a->A.ball, *ball);
+    cc_arena_free(a);
+    return mode;
+}

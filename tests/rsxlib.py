@@ -146,6 +146,11 @@ def load_cpucheck():
         ("cc_point_tri_dist2", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
         ("cc_env", C.c_int, [VP, C.POINTER(C.c_float), C.c_int, C.c_float, C.POINTER(C.c_float),
                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        ("cc_train_rewards", None, [VP, C.POINTER(CarState), C.POINTER(CarState), C.c_int, C.POINTER(BallState),
+                                    C.POINTER(BallState), C.POINTER(engine.TrainingRewardEntry), C.c_int, C.c_int,
+                                    C.c_uint64, C.c_int, C.POINTER(C.c_float)]),
+        ("cc_curriculum_reset", C.c_int, [VP, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_float),
+                                          C.POINTER(CarState), C.POINTER(BallState)]),
     ])
 
 
