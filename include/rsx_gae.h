@@ -40,15 +40,15 @@ typedef struct rsx_gae_params {
  * target values (value + advantage), returns (the discounted reward sum
  * within an episode); d_rew_clip_portion (nullable, one float): the share of
  * |standardized reward| removed by the clip.  Uses a grow-only scratch buffer
- * per device (about 16 bytes per 1024 steps), allocated on first use or
+ * per device (24 bytes per 8192 steps), allocated on first use or
  * growth (then the device is synchronized). */
 rsx_status rsx_gae(const float* d_rews, const int8_t* d_terminals, const float* d_val_preds,
                    const float* d_trunc_val_preds, int32_t num_truncs, float* d_advantages, float* d_target_values,
                    float* d_returns, float* d_rew_clip_portion, int32_t T, const rsx_gae_params* params,
                    void* stream);
 
-/* Largest T that rsx_gae_noalloc takes (its fixed scratch: 16 bytes per
- * 1024 steps, 1 MiB per GPU). */
+/* Largest T that rsx_gae_noalloc takes (its fixed scratch: 24 bytes per
+ * 8192 steps, 192 KiB per GPU). */
 #define RSX_GAE_NOALLOC_MAX_STEPS (1 << 26)
 
 /* No counterpart in the reference: rsx_gae for callers that cannot allocate

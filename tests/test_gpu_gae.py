@@ -41,9 +41,13 @@ def _device(r, term, v, tv, **kw):
     return [o.cpu().numpy() for o in out]
 
 
-CASES = [(1, 0.0, 0.0, None), (5, 0.2, 0.2, 0), (1023, 0.01, 0.01, 1), (1024, 0.003, 0.003, 2),
-         (1025, 0.0, 0.0, 0), (3000, 0.0, 0.0, 2), (4096, 0.5, 0.5, None), (20000, 0.004, 0.002, 0),
-         (100003, 0.001, 0.0005, 2)]
+# sizes around a lane's 16 steps and a workgroup's 8192-step region; no
+# terminal at all (one segment over every region), all terminals, a last
+# step with and without one
+CASES = [(1, 0.0, 0.0, None), (5, 0.2, 0.2, 0), (16, 0.1, 0.1, 0), (17, 0.0, 0.0, 2), (1023, 0.01, 0.01, 1),
+         (1024, 0.003, 0.003, 2), (1025, 0.0, 0.0, 0), (3000, 0.0, 0.0, 2), (4096, 0.5, 0.5, None),
+         (8191, 0.001, 0.001, 0), (8192, 0.0, 0.0, 1), (8193, 0.002, 0.0, 0), (20000, 0.004, 0.002, 0),
+         (3 * 8192 + 5, 0.0001, 0.0, 2), (100003, 0.001, 0.0005, 2)]
 
 
 @pytest.mark.parametrize("T,pn,pt,last", CASES)
