@@ -204,6 +204,40 @@ def cpu_baseline(seconds, threads, workload, tick_skip=8, sample_every=32):
                                         f"{k_dt:.1f} s"}}
 
 
+def gae_bench(eng, n_players, dev, steps_per_player=64, reps=20):
+    """The device GAE (rsx_gae) over one training iteration's experience
+    buffer: every player's trajectory of `steps_per_player` steps ending in a
+    truncation, goals scattered (p = 0.002 per step).  Algorithmic bytes per
+    step: reward, value, terminal in (9 B), advantage, target, return out
+    (12 B)."""
+    import torch
+    from emy_s_framework_amd.gae import ComputeGAEOnDevice
+    T = n_players * steps_per_player
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    term = torch.zeros(T, dtype=torch.int8, device=dev)
+    term[steps_per_player - 1::steps_per_player] = 2
+    goal = (torch.rand(T, device=dev, generator=g) < 0.002) & (term == 0)
+    term[goal] = 1
+    rews = torch.randn(T, device=dev, generator=g)
+    vals = torch.randn(T, device=dev, generator=g)
+    tv = torch.randn(n_players, device=dev, generator=g)
+    for _ in range(3):
+        ComputeGAEOnDevice(rews, term, vals, tv)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ComputeGAEOnDevice(rews, term, vals, tv)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = T * 21 / (ms * 1e-3) / 1e9
+    print(f"# device GAE: {T} steps in {ms:.3f} ms ({gbs:.0f} GB/s)", file=sys.stderr, flush=True)
+    return {"steps": T, "layout": f"{n_players} players x {steps_per_player} steps, each trajectory truncated at its "
+                                  f"end, goals p=0.002", "ms": ms, "steps_per_s": T / (ms * 1e-3),
+            "achieved_gbs": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS, "algo_bytes_per_step": 21}
+
+
 def measured_traffic(workload):
     """HBM bytes per k_env_step<2> launch from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-scaled),
@@ -232,7 +266,13 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape 'block,arenas_per_wave[,pool]' (skips the autotuner)")
     ap.add_argument("--cull", choices=("auto", "on", "off"), default="auto")
-    ap.add_argument("--audit-steps", type=int, default=8)
+    ap.add_argument("--audit-steps", type=int, default=1000,
+                    help="env steps of the culled engine with every culled body checked against the exact mesh "
+                         "distance on every tick")
+    ap.add_argument("--long-steps", type=int, default=300,
+                    help="env steps of the long window (exact and culled, from fresh states); 0 skips it")
+    ap.add_argument("--gae", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the device GAE on the buffer of one training iteration")
     ap.add_argument("--cull-every", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -342,20 +382,30 @@ def main():
                 "launch_shape": {"block": shape[0], "arenas_per_wave": shape[1], "pool_narrowphase": shape[2]},
                 "autotune": tuning}
 
+    def long_window(tag):
+        if args.long_steps <= 0:
+            return None
+        r = run(tag, args.long_steps, 0, retune=False)
+        return {"steps": args.long_steps, "ms_per_step": r["ms_per_step"], "kernel_ms": r["kernel_ms"],
+                "value": r["value"]}
+
     exact = run("exact collision", args.steps, args.warmup)
-    cull = audit = None
+    exact_long = long_window("exact collision, long window")
+    cull = audit = cull_long = None
     if use_cull:
         net.attach(margin[0], every=args.cull_every, audit=False, margin_car_uu=margin[1])
         cull = run("learned cull", args.steps, args.warmup)
+        cull_long = long_window("learned cull, long window")
         net.attach(margin[0], every=args.cull_every, audit=True, margin_car_uu=margin[1])
         net.reset_stats()
         run("learned cull, audited", args.audit_steps, 0, retune=False)
         audit = net.stats()
         body_ticks = args.audit_steps * args.tick_skip * (args.arenas + n_cars)
-        audit.update({"steps": args.audit_steps, "margin_ball_uu": margin[0], "margin_car_uu": margin[1],
-                      "every_ticks": args.cull_every,
+        audit.update({"steps": args.audit_steps, "ticks_audited": args.audit_steps * args.tick_skip,
+                      "margin_ball_uu": margin[0], "margin_car_uu": margin[1], "every_ticks": args.cull_every,
                       "culled_fraction": audit["culled"] / body_ticks})
         net.detach()
+    gae = gae_bench(eng, n_cars, dev) if args.gae else None
     head = cull if cull is not None else exact
     achieved = algo_bytes / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
@@ -390,7 +440,10 @@ def main():
                             f"fallback" if cull is not None else "none (exact collision)")},
         "exact": exact,
         "cull": cull,
+        "exact_long": exact_long,
+        "cull_long": cull_long,
         "audit": audit,
+        "gae": gae,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload) if args.arenas == 65536 else None,
