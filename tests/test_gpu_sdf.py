@@ -169,3 +169,65 @@ def test_cull_in_tick_is_exact_and_audited(torch_cuda):
         for t in range(len(r0.balls)):
             assert bytes(r0.balls[t]) == bytes(r1.balls[t])
             assert [bytes(c) for c in r0.cars[t]] == [bytes(c) for c in r1.cars[t]]
+
+
+def test_cull_work_skipped_and_time_on_stress_states(torch_cuda):
+    """What the trained network gains over the exact engine on the
+    contact-heavy workload (65536 1v1 arenas, seed 4, the cull every 8 ticks
+    with the committed margins): the same 16 fused env steps from the same
+    states and actions with and without it give bit-identical arenas, an
+    audited run of the culled engine finds no culled body within reach of the
+    mesh, and the share of body-ticks culled and both step times are
+    reported (measured, not gated: on this workload they are equal to within
+    the box's run-to-run spread)."""
+    torch = torch_cuda
+    eng = R.import_engine()
+    from emy_s_framework_amd.sdf import WEIGHTS_PATH, NeuralSdf, margins_for
+    from emy_s_framework_amd.workloads import stress_states
+    if not os.path.exists(WEIGHTS_PATH):
+        pytest.skip("no trained weights committed")
+    n, steps = 65536, 16
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(4)
+    acts = [torch.randint(0, 90, (2 * n,), dtype=torch.int32, device="cuda", generator=gen) for _ in range(steps)]
+
+    def run(audit=None):
+        """A fresh batch (each arena's RNG stream and carried wheel values
+        start alike), the workload, `steps` env steps; the cull attached
+        unless audit is None."""
+        b = eng.RocketSimBatch(n, 1, seed=4)
+        net = None
+        try:
+            b.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
+            b.SetTuning(128, 64, 1)
+            if audit is not None:
+                net = NeuralSdf(b, seed=0)
+                mb, mc = margins_for(net.load(WEIGHTS_PATH), 8)
+                net.attach(mb, every=8, audit=audit, margin_car_uu=mc)
+                net.reset_stats()
+            stress_states(b, 4)
+            b.Synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for a in acts:
+                b.StepEnv(a.data_ptr())
+            e1.record()
+            torch.cuda.synchronize()
+            st = net.stats() if net is not None else None
+            return bytes(b.GetAllCarStates()) + bytes(b.GetAllBallStates()), e0.elapsed_time(e1) / steps, st
+        finally:
+            if net is not None:
+                net.detach()
+                net.destroy()
+            b.Destroy()
+
+    exact, t_exact, _ = run()
+    culled, t_cull, st = run(audit=False)
+    audited, _, st_audit = run(audit=True)
+    assert culled == exact and audited == exact
+    assert st_audit["defects"] == 0, st_audit
+    assert st["culled"] > 0, st
+    frac = st["culled"] / (steps * 8 * (n + 2 * n))
+    print(f"65536 contact-heavy 1v1 arenas x {steps} env steps: exact {t_exact:.2f} ms/step, culled "
+          f"{t_cull:.2f} ms/step; {100 * frac:.1f}% of body-ticks culled, {st['queries']} network queries; "
+          f"audit {st_audit}", flush=True)
