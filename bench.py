@@ -238,6 +238,45 @@ def gae_bench(eng, n_players, dev, steps_per_player=64, reps=20):
             "achieved_gbs": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS, "algo_bytes_per_step": 21}
 
 
+def vanta_bench(batch, apply_states, stream, sp, actions, reps=5):
+    """VantaObs (rsx_build_obs with RSX_OBS_VANTA, ball prediction on) over
+    the whole batch: the Vanta kernel alone, and a fused env step followed by
+    it, timed with HIP events on the launch stream.  Restores DefaultObs."""
+    import torch
+    eng = _import_package()
+    apply_states(batch, seed=4)
+    batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True, obs_builder=eng.OBS_VANTA)
+    batch.ConfigureVanta(predict_ball=True)
+    batch.BuildObs(sp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        batch.BuildObs(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    build_ms = e0.elapsed_time(e1) / reps
+    e0.record(stream)
+    for i in range(reps):
+        batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / reps
+    batch.ConfigureVanta(predict_ball=False)
+    e0.record(stream)
+    for _ in range(reps):
+        batch.BuildObs(sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    nopred_ms = e0.elapsed_time(e1) / reps
+    batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
+    print(f"# VantaObs: build {build_ms:.2f} ms (no prediction {nopred_ms:.2f} ms), env step + build {step_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+    return {"build_ms": build_ms, "build_ms_no_prediction": nopred_ms, "env_step_plus_build_ms": step_ms,
+            "env_steps_per_s": batch.num_arenas / (step_ms * 1e-3),
+            "rollout": "360 ball-only ticks per arena (slices at 1, 2, 3 s), every build"}
+
+
 def measured_traffic(workload):
     """HBM bytes per k_env_step<2> launch from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-scaled),
@@ -273,6 +312,8 @@ def main():
                     help="env steps of the long window (exact and culled, from fresh states); 0 skips it")
     ap.add_argument("--gae", action=argparse.BooleanOptionalAction, default=True,
                     help="also time the device GAE on the buffer of one training iteration")
+    ap.add_argument("--vanta", action=argparse.BooleanOptionalAction, default=True,
+                    help="also time the VantaObs build (ball prediction on) over the same arenas")
     ap.add_argument("--cull-every", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -406,6 +447,7 @@ def main():
                       "culled_fraction": audit["culled"] / body_ticks})
         net.detach()
     gae = gae_bench(eng, n_cars, dev) if args.gae else None
+    vanta = vanta_bench(batch, apply_states, stream, sp, actions) if args.vanta else None
     head = cull if cull is not None else exact
     achieved = algo_bytes / (head["kernel_ms"] * 1e-3) / 1e9
     cpu = None
@@ -444,6 +486,7 @@ def main():
         "cull_long": cull_long,
         "audit": audit,
         "gae": gae,
+        "vanta_obs": vanta,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.workload) if args.arenas == 65536 else None,

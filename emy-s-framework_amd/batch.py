@@ -9,7 +9,7 @@ import os
 
 from .records import (ArenaInfo, BallState, BoostPadState, CarControls, CarState, Config, EnvConfig, NUM_PADS,
                       PadStates, CARS_BLUE_FIRST, REWARD, MAX_TRAINING_REWARDS, TrainingRewardEntry, Tuning,
-                      TuneResult)
+                      TuneResult, VantaConfig)
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "librsx.so")
@@ -69,6 +69,9 @@ _SIGS = [
     ("rsx_get_tuning", _I, [_VP, C.POINTER(Tuning)]),
     ("rsx_autotune", _I, [_VP, _VP, _I, C.POINTER(TuneResult), _I, C.POINTER(_I)]),
     ("rsx_capture_env_graph", _I, [_VP, _VP, _I]),
+    ("rsx_configure_vanta", _I, [_VP, C.POINTER(VantaConfig)]),
+    ("rsx_get_vanta_config", _I, [_VP, C.POINTER(VantaConfig)]),
+    ("rsx_get_vanta_prediction", _I, [_VP, _I, C.POINTER(BallState), C.POINTER(_I)]),
     ("rsx_launch_env_graph", _I, [_VP, _VP]),
     # learned broadphase cull (include/rsx_sdf.h)
     ("rsx_sdf_num_params", _I, []),
@@ -265,8 +268,9 @@ class RocketSimBatch:
                      no_touch_timeout_s=0.0, goal_terminal=True, auto_reset=True, obs_builder=0,
                      no_touch_stagger=False):
         """ConfigureTrainingTerminals and the stub rewards' weights;
-        obs_builder 0 (DefaultObs) or 1 (AdvancedObs,
-        BuildAdvancedObsAndDefaultMasks); no_touch_stagger: resets start the
+        obs_builder 0 (DefaultObs), 1 (AdvancedObs,
+        BuildAdvancedObsAndDefaultMasks) or 2 (VantaObs, BuildVantaObs;
+        see ConfigureVanta); no_touch_stagger: resets start the
         no-touch timer at a random age, as the reference's ResetArena."""
         c = EnvConfig(goal, concede, touch, vel_ball_to_goal, vel_player_to_ball, no_touch_timeout_s,
                       int(goal_terminal), int(auto_reset), int(obs_builder), int(no_touch_stagger))
@@ -308,6 +312,26 @@ class RocketSimBatch:
         w = (C.c_float * 4)()
         _check(self._lib.rsx_get_reset_curriculum(self._h, w))
         return list(w)
+
+    def ConfigureVanta(self, predict_ball=False, only_closest_opp=False, dodge_deadzone=0.5, tick_skip=0):
+        """The VantaObs parameters of BatchConfig (vantaPredictBall,
+        vantaOnlyClosestOpp, vantaDodgeDeadzone, vantaTickSkip;
+        RocketSimCuda.h:184-189); tick_skip 0 uses the engine's."""
+        c = VantaConfig(int(bool(predict_ball)), int(bool(only_closest_opp)), float(dodge_deadzone), int(tick_skip))
+        _check(self._lib.rsx_configure_vanta(self._h, C.byref(c)))
+
+    def GetVantaConfig(self):
+        c = VantaConfig()
+        _check(self._lib.rsx_get_vanta_config(self._h, C.byref(c)))
+        return c
+
+    def GetVantaPrediction(self, arena):
+        """The ball-prediction slices (3 BallStates, uu) of the arena's last
+        Vanta build, or None when it had none."""
+        out = (BallState * 3)()
+        has = C.c_int32(0)
+        _check(self._lib.rsx_get_vanta_prediction(self._h, arena, out, C.byref(has)))
+        return list(out) if has.value else None
 
     def StepEnv(self, device_actions_ptr, stream=None):
         """Fused env step from a device int32 array of num_cars action indices."""

@@ -65,6 +65,7 @@ struct Bucket {
     int64_t* d_obs_off = nullptr;    // n: float offset of each arena's first obs row
     int* d_global = nullptr;         // n: global arena index
     std::vector<int> global;         // bucket-local -> global arena index
+    uint32_t* d_vanta = nullptr;     // VantaObs state words (field-major), allocated with the first Vanta config
 };
 
 struct rsx_engine {
@@ -102,6 +103,10 @@ struct rsx_engine {
     RewardEntry* d_rt = nullptr;
     int rt_n = 0;
     std::vector<rsx_training_reward_entry> rt_host;
+    // VantaObs (rsx_configure_vanta): parameters, pad tables, state allocated
+    vanta::Params vq{8.f / 120.f, 0.5f, 0, 0};
+    vanta::PadTables vpads{};
+    bool vanta_alloc = false;
     // captured env steps (rsx_capture_env_graph)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -299,6 +304,73 @@ hipError_t env_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, h
     return hipSuccess;
 }
 
+// VantaObs rows of every arena (BuildVantaObs); `reset`: per-arena bytes
+// whose non-zero entries mark arenas reset since the last build (their
+// Vanta state is cleared first), or null.
+hipError_t vanta_launch(rsx_engine* e, const uint8_t* reset, hipStream_t st) {
+    VantaArgs v;
+    v.cg = e->hw.cg;
+    v.bg = e->hw.bg;
+    v.mv = e->dview;
+    v.pads = e->dw.pads;
+    v.P = e->vpads;
+    v.Q = e->vq;
+    v.obs = e->d_obs;
+    for (const Bucket& b : e->bucket) {
+        if (!b.n) continue;
+        const hipError_t r = launch_vanta(b.maxc, bucket_args(b), b.d_vanta, reset, v, st);
+        if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+}
+
+// One env step as rsx_step_env runs it: the fused launch, then (with
+// VantaObs) the Vanta rows, clearing the state of the arenas the step reset.
+hipError_t env_step_launch(rsx_engine* e, const int32_t* act, const LaunchShape& s, hipStream_t st) {
+    hipError_t r = env_launch(e, act, s, st);
+    if (r == hipSuccess && e->env.obs_kind == kObsVanta)
+        r = vanta_launch(e, e->env.auto_reset ? e->d_term : nullptr, st);
+    return r;
+}
+
+// Pad tables of VantaObs (RocketSimCudaHostUtils.cuh:68-86): each RLGym pad
+// slot's arena pad and location, and the slot of its x-mirror.
+vanta::PadTables vanta_pad_tables(const PadGeom* pads, const int8_t* pad_map) {
+    vanta::PadTables t{};
+    for (int i = 0; i < vanta::kPads; i++) {
+        t.map[i] = pad_map[i];
+        t.loc[i] = pads[pad_map[i]].pos_uu;
+    }
+    for (int i = 0; i < vanta::kPads; i++) {
+        int match = i;
+        for (int j = 0; j < vanta::kPads; j++)
+            if (fabsf(t.loc[j].x + t.loc[i].x) < 1.f && fabsf(t.loc[j].y - t.loc[i].y) < 1.f &&
+                fabsf(t.loc[j].z - t.loc[i].z) < 1.f) {
+                match = j;
+                break;
+            }
+        t.mirror[i] = (int8_t)match;
+    }
+    return t;
+}
+
+int vanta_words(int maxc) {
+    return maxc == 2 ? vanta::VLayout<2>::kWords : (maxc == 4 ? vanta::VLayout<4>::kWords : vanta::VLayout<6>::kWords);
+}
+
+// Allocates every bucket's Vanta words, all zero (no state yet).
+rsx_status vanta_alloc(rsx_engine* e) {
+    if (e->vanta_alloc) return RSX_OK;
+    for (Bucket& b : e->bucket) {
+        if (!b.n) continue;
+        const size_t bytes = sizeof(uint32_t) * (size_t)vanta_words(b.maxc) * b.n;
+        HIP_TRY(hipMalloc((void**)&b.d_vanta, bytes));
+        HIP_TRY(hipMemset(b.d_vanta, 0, bytes));
+    }
+    e->vanta_alloc = true;
+    return RSX_OK;
+}
+
 rsx_status column_get(rsx_engine* e, int arena, std::vector<uint32_t>& out, Bucket*& b, int& local) {
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (arena < 0 || arena >= e->num_arenas) return fail(RSX_INVALID_ARG, "arena index out of range");
@@ -436,9 +508,9 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         E.no_touch_stagger = 0;
         HIP_TRY(hipMalloc((void**)&e->d_rt, sizeof(RewardEntry) * kMaxRewardEntries));
         if (!obs_pad_map(e->hw.pads, E.pad_map)) return fail(RSX_MESH_LOAD_FAILED, "boost pad layout does not match");
-        // the obs buffer holds the longer (AdvancedObs) rows, so its address
+        // the obs buffer holds the longest (VantaObs) rows, so its address
         // survives a change of observation builder
-        obs_layout_for(e.get(), kObsAdvanced);
+        obs_layout_for(e.get(), kObsVanta);
         const int64_t off = e->obs_floats;
         obs_layout_for(e.get(), kObsDefault);
         HIP_TRY(hipMalloc((void**)&e->d_obs, sizeof(float) * std::max<int64_t>(1, off)));
@@ -448,6 +520,8 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         HIP_TRY(hipMemset(e->d_rew, 0, sizeof(float) * (size_t)e->num_cars));
         HIP_TRY(hipMemset(e->d_term, 0, (size_t)e->num_arenas));
         if ((s = upload(&e->d_mask_tables, default_action_masks())) != RSX_OK) return s;
+        e->vpads = vanta_pad_tables(e->hw.pads, E.pad_map);
+        e->vq.dt = (float)e->tick_skip / 120.f;
     }
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
@@ -497,6 +571,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
         (void)hipFree(b.d_car_off);
         (void)hipFree(b.d_obs_off);
         (void)hipFree(b.d_global);
+        (void)hipFree(b.d_vanta);
     }
     (void)hipFree(e->d_obs);
     (void)hipFree(e->d_rew);
@@ -656,7 +731,13 @@ rsx_status rsx_reset_arena(rsx_engine* e, int32_t arena) {
     if (b->maxc == 2) reset_column<2>(c, e->hw.cg, &e->env);
     else if (b->maxc == 4) reset_column<4>(c, e->hw.cg, &e->env);
     else reset_column<6>(c, e->hw.cg, &e->env);
-    return column_put(e, b, local, col);
+    RSX_TRY(column_put(e, b, local, col));
+    if (e->vanta_alloc) {
+        // ObsBuilder::Reset: the arena's VantaObs state starts over
+        HIP_TRY(launch_vanta_clear(b->maxc, b->d_vanta, b->n, local, nullptr));
+        HIP_TRY(hipStreamSynchronize(nullptr));
+    }
+    return RSX_OK;
 }
 
 rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream) {
@@ -673,6 +754,7 @@ rsx_status rsx_reset_all_arenas(rsx_engine* e, void* stream) {
             hipLaunchKernelGGL(k_reset<4>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg, e->env);
         else hipLaunchKernelGGL(k_reset<6>, grid, block, 0, st, b.d_words, b.n, nullptr, e->hw.cg, e->env);
         HIP_TRY(hipGetLastError());
+        if (e->vanta_alloc) HIP_TRY(launch_vanta_clear(b.maxc, b.d_vanta, b.n, -1, st));
     }
     return RSX_OK;
 }
@@ -936,8 +1018,12 @@ rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c) {
     E.goal_terminal = c->goal_terminal != 0;
     E.auto_reset = c->auto_reset != 0;
     E.no_touch_stagger = c->no_touch_stagger != 0;
-    if (c->obs_builder != RSX_OBS_DEFAULT && c->obs_builder != RSX_OBS_ADVANCED)
-        return fail(RSX_INVALID_ARG, "obs_builder must be RSX_OBS_DEFAULT or RSX_OBS_ADVANCED");
+    if (c->obs_builder != RSX_OBS_DEFAULT && c->obs_builder != RSX_OBS_ADVANCED && c->obs_builder != RSX_OBS_VANTA)
+        return fail(RSX_INVALID_ARG, "obs_builder must be RSX_OBS_DEFAULT, RSX_OBS_ADVANCED or RSX_OBS_VANTA");
+    if (c->obs_builder == RSX_OBS_VANTA) {
+        HIP_TRY(hipSetDevice(e->device));
+        RSX_TRY(vanta_alloc(e));
+    }
     if (c->obs_builder != E.obs_kind) {
         HIP_TRY(hipSetDevice(e->device));
         HIP_TRY(hipDeviceSynchronize());  // no env step may still read the old offsets
@@ -1029,7 +1115,7 @@ rsx_status rsx_step_env(rsx_engine* e, const int32_t* device_actions, void* stre
     if (!device_actions) return fail(RSX_INVALID_ARG, "null actions");
     HIP_TRY(hipSetDevice(e->device));
     RSX_TRY(prepare_shape(e, e->shape));
-    HIP_TRY(env_launch(e, device_actions, e->shape, (hipStream_t)stream));
+    HIP_TRY(env_step_launch(e, device_actions, e->shape, (hipStream_t)stream));
     return RSX_OK;
 }
 
@@ -1037,9 +1123,11 @@ rsx_status rsx_build_obs(rsx_engine* e, void* stream) {
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     HIP_TRY(hipSetDevice(e->device));
     hipStream_t st = (hipStream_t)stream;
+    const bool vanta_on = e->env.obs_kind == kObsVanta;
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
         if (!b.n) continue;
+        if (vanta_on && !e->masks_on) continue;  // the Vanta kernel writes the rows
         dim3 grid((b.n + 63) / 64), block(64);
         uint8_t* masks = e->masks_on ? e->d_masks : nullptr;
 #define RSX_LAUNCH_OBS(M)                                                                                        \
@@ -1051,6 +1139,7 @@ rsx_status rsx_build_obs(rsx_engine* e, void* stream) {
 #undef RSX_LAUNCH_OBS
         HIP_TRY(hipGetLastError());
     }
+    if (vanta_on) HIP_TRY(vanta_launch(e, nullptr, st));
     return RSX_OK;
 }
 
@@ -1275,7 +1364,7 @@ rsx_status rsx_capture_env_graph(rsx_engine* e, const int32_t* device_actions, i
     hipStream_t cs = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipError_t r = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    for (int i = 0; i < env_steps && r == hipSuccess; i++) r = env_launch(e, device_actions, e->shape, cs);
+    for (int i = 0; i < env_steps && r == hipSuccess; i++) r = env_step_launch(e, device_actions, e->shape, cs);
     hipGraph_t g = nullptr;
     const hipError_t r_end = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -1298,6 +1387,53 @@ rsx_status rsx_launch_env_graph(rsx_engine* e, void* stream) {
         return fail(RSX_INVALID_ARG, "the launch shape, env config, action table or cull changed since the capture");
     HIP_TRY(hipSetDevice(e->device));
     HIP_TRY(hipGraphLaunch(e->graph_exec, (hipStream_t)stream));
+    return RSX_OK;
+}
+
+rsx_status rsx_configure_vanta(rsx_engine* e, const rsx_vanta_config* c) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!c) return fail(RSX_INVALID_ARG, "null Vanta config");
+    if (c->tick_skip < 0) return fail(RSX_INVALID_ARG, "tick_skip must be >= 0");
+    if (!(c->dodge_deadzone >= 0.f && c->dodge_deadzone <= 1.f))
+        return fail(RSX_INVALID_ARG, "dodge_deadzone must lie in [0, 1]");
+    e->vq.predict = c->predict_ball != 0;
+    e->vq.only_closest_opp = c->only_closest_opp != 0;
+    e->vq.deadzone = c->dodge_deadzone;
+    e->vq.dt = (float)(c->tick_skip > 0 ? c->tick_skip : e->tick_skip) / 120.f;
+    e->gen++;
+    return RSX_OK;
+}
+
+rsx_status rsx_get_vanta_config(const rsx_engine* e, rsx_vanta_config* c) {
+    if (!e || !c) return fail(RSX_INVALID_ARG, "null argument");
+    c->predict_ball = e->vq.predict;
+    c->only_closest_opp = e->vq.only_closest_opp;
+    c->dodge_deadzone = e->vq.deadzone;
+    c->tick_skip = (int32_t)lroundf(e->vq.dt * 120.f);
+    return RSX_OK;
+}
+
+rsx_status rsx_get_vanta_prediction(rsx_engine* e, int32_t arena, rsx_ball_state* out3, int32_t* has_prediction) {
+    if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
+    if (!out3 || !has_prediction) return fail(RSX_INVALID_ARG, "null argument");
+    if (arena < 0 || arena >= e->num_arenas) return fail(RSX_INVALID_ARG, "arena index out of range");
+    *has_prediction = 0;
+    memset(out3, 0, 3 * sizeof(rsx_ball_state));
+    if (!e->vanta_alloc) return RSX_OK;
+    Bucket& b = e->bucket[e->arena_bucket[arena]];
+    const int local = e->arena_local[arena];
+    std::vector<uint32_t> col(vanta::kVaWords);
+    HIP_TRY(hipSetDevice(e->device));
+    HIP_TRY(hipMemcpy2D(col.data(), sizeof(uint32_t), b.d_vanta + local, sizeof(uint32_t) * b.n, sizeof(uint32_t),
+                        vanta::kVaWords, hipMemcpyDeviceToHost));
+    const Col c{col.data(), 1};
+    *has_prediction = (c.get(vanta::kVaAvailHi) >> 9) & 1;
+    for (int k = 0; k < 3; k++) {
+        const V3 p = c.getv(vanta::kVaPredPos + 3 * k), v = c.getv(vanta::kVaPredVel + 3 * k);
+        out3[k].pos = rsx_vec3{p.x, p.y, p.z};
+        out3[k].vel = rsx_vec3{v.x, v.y, v.z};
+        out3[k].ang_vel = rsx_vec3{0.f, 0.f, 0.f};
+    }
     return RSX_OK;
 }
 

@@ -31,8 +31,11 @@ RSX_HD int obs_row_len(int ncars) { return kObsHead + kPlayerObs * ncars; }
 // Observation builders of the env step.
 constexpr int kObsDefault = 0;   // RLGym DefaultObs: 51 + 19 per car
 constexpr int kObsAdvanced = 1;  // GigaLearn AdvancedObs as the GPU path builds it: 51 + 29 per car
+constexpr int kObsVanta = 2;     // VantaObs v3.5: 366 per car whatever the team size (rsx_vanta.h)
 constexpr int kAdvPlayerObs = 29;
+constexpr int kVantaRow = 366;
 RSX_HD int obs_row_len_of(int kind, int ncars) {
+    if (kind == kObsVanta) return kVantaRow;
     return kObsHead + (kind == kObsAdvanced ? kAdvPlayerObs : kPlayerObs) * ncars;
 }
 
@@ -227,11 +230,13 @@ RSX_HD void adv_obs_row(const Arena<MAXC>& A, int self, const EnvConfig& E, floa
 
 // The configured observation row of car `self`; prev is DefaultObs's
 // previous-action block (AdvancedObs reads the car's last controls instead).
+// VantaObs rows are written by their own kernel after the step (they carry
+// per-arena state and a ball-only rollout), so nothing is written here.
 template <int MAXC>
 RSX_HD void env_obs_row(const Arena<MAXC>& A, int self, const float (*prev)[kActionFloats], const EnvConfig& E,
                         float* o) {
     if (E.obs_kind == kObsAdvanced) adv_obs_row<MAXC>(A, self, E, o);
-    else obs_row<MAXC>(A, self, prev, E, o);
+    else if (E.obs_kind == kObsDefault) obs_row<MAXC>(A, self, prev, E, o);
 }
 
 RSX_HD bool touched_in_step(const Car& c, uint64_t tick, int tick_skip) {

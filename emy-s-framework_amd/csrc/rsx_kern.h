@@ -26,6 +26,7 @@
 #include "../../include/rsx_types.h"
 #include "rsx_train.h"
 #include "rsx_sdf.h"
+#include "rsx_vanta.h"
 
 namespace rsx {
 
@@ -96,6 +97,24 @@ hipError_t launch_env(int maxc, const BucketArgs& b, const WorldArgs& w, const E
 // 3v3 wavefronts come first in the grid, so the longest work starts first.
 hipError_t launch_env_mixed(const BucketArgs* b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
                             hipStream_t st);
+// VantaObs rows of one bucket (k_vanta, rsx_vanta.hip): one lane per arena
+// clears the arena's Vanta words when reset[global] is non-zero (reset may
+// be null), runs the ball prediction when asked, updates the pad / demo and
+// coyote timers and writes every car's 366-float row at obs + obs_off.
+struct VantaArgs {
+    CarGeom cg;
+    BallGeom bg;
+    MeshView mv;
+    const PadGeom* pads;
+    vanta::PadTables P;
+    vanta::Params Q;
+    float* obs;
+};
+hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const uint8_t* reset, const VantaArgs& v,
+                        hipStream_t st);
+// Clears the Vanta words of every arena of a bucket (only < 0) or of arena `only`.
+hipError_t launch_vanta_clear(int maxc, uint32_t* vwords, int n, int only, hipStream_t st);
+
 // Sets each arena kernel's dynamic-LDS limit to `bytes` (needed above 64 KiB);
 // call outside stream capture.
 hipError_t prepare_arena_kernels(size_t bytes);

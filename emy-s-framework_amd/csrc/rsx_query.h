@@ -154,25 +154,93 @@ RSX_HD bool ray_triangle(V3 from, V3 to, V3 v0, V3 v1, V3 v2, float& hit_fractio
     return true;
 }
 
+// The ray test of btTriangleRaycastCallback::processTriangle without the
+// closest-hit comparison: the crossing fraction and hit normal of the
+// triangle, or false when the ray misses it.
+RSX_HD bool ray_triangle_hit(V3 from, V3 to, V3 v0, V3 v1, V3 v2, float& distance, V3& hit_normal) {
+    V3 v10 = v1 - v0;
+    V3 v20 = v2 - v0;
+    V3 tn = cross(v10, v20);
+    const float d = dot(v0, tn);
+    float da = dot(tn, from);
+    da -= d;
+    float db = dot(tn, to);
+    db -= d;
+    if (da * db >= 0.0f) return false;
+    const float proj = da - db;
+    distance = da / proj;
+    float tol = len2(tn);
+    tol *= -0.0001f;
+    float s = 1.0f - distance;
+    V3 point = V3{s * from.x + distance * to.x, s * from.y + distance * to.y, s * from.z + distance * to.z};
+    V3 v0p = v0 - point;
+    V3 v1p = v1 - point;
+    V3 cp0 = cross(v0p, v1p);
+    if (!(dot(cp0, tn) >= tol)) return false;
+    V3 v2p = v2 - point;
+    V3 cp1 = cross(v1p, v2p);
+    if (!(dot(cp1, tn) >= tol)) return false;
+    V3 cp2 = cross(v2p, v0p);
+    if (!(dot(cp2, tn) >= tol)) return false;
+    tn = normalized(tn);
+    hit_normal = (da <= 0.0f) ? -tn : tn;
+    return true;
+}
+
+// The mesh triangles near a ray, tested as the grid walk meets them.  Bullet
+// visits them in tree order and keeps a hit only when it is strictly closer,
+// so the hit it reports is the one with the smallest fraction, the first in
+// tree order among equals; keeping the lexicographically smallest
+// (fraction, packed key) as the walk finds them gives the same hit without
+// listing and sorting the candidates.  Triangles listed in several cells are
+// tested once, from the first such cell the walk visits.
+RSX_HD void ray_mesh(const MeshView& w, V3 from, V3 to, RayHit& hit) {
+    const V3 mn = vmin(from, to), mx = vmax(from, to);
+    const float inv = 1.0f / w.grid_cell;
+    const int x0 = clampi((int)floorf((mn.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y0 = clampi((int)floorf((mn.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z0 = clampi((int)floorf((mn.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    const int x1 = clampi((int)floorf((mx.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y1 = clampi((int)floorf((mx.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z1 = clampi((int)floorf((mx.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    float frac = hit.object >= 0 ? hit.fraction : 1.0f;
+    int best = -1;  // packed key of the hit kept so far (-1: none from the mesh)
+    for (int ix = x0; ix <= x1; ix++)
+        for (int iy = y0; iy <= y1; iy++)
+            for (int iz = z0; iz <= z1; iz++) {
+                const int cell = (ix * w.gy + iy) * w.gz + iz;
+                const int s = w.cell_start[cell], e = w.cell_start[cell + 1];
+                for (int k = s; k < e; k++) {
+                    const GridEntry g = w.cell_ent[k];
+                    const int fx = maxi(x0, g.first & 1023);
+                    const int fy = maxi(y0, (g.first >> 10) & 1023);
+                    const int fz = maxi(z0, g.first >> 20);
+                    if (fx != ix || fy != iy || fz != iz) continue;
+                    if (g.mn[2] > mx.z || g.mx[2] < mn.z) continue;
+                    if (g.mn[0] > mx.x || g.mx[0] < mn.x) continue;
+                    if (g.mn[1] > mx.y || g.mx[1] < mn.y) continue;
+                    V3 a, b, c, nrm;
+                    load_tri(w, cand_tri(g.key), a, b, c);
+                    float d;
+                    if (!ray_triangle_hit(from, to, a, b, c, d, nrm)) continue;
+                    if (d < frac || (d == frac && best >= 0 && g.key < best)) {
+                        frac = d;
+                        best = g.key;
+                        hit.fraction = d;
+                        hit.normal = nrm;
+                        hit.object = 0;
+                    }
+                }
+            }
+}
+
 // Full Bullet ray test against the static world (mesh triangles near the ray,
 // then the four planes), closest hit wins (strictly closer replaces).
 // with_mesh = false leaves out the mesh triangles, for a ray the learned cull
 // has shown to be out of their reach.
 RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit, bool with_mesh = true) {
-    V3 mn = vmin(from, to), mx = vmax(from, to);
-    int cand[kMaxCand];
-    int n = with_mesh ? gather_tris(w, mn, mx, cand, kMaxCand) : 0;
+    if (with_mesh) ray_mesh(w, from, to, hit);
     float frac = hit.object >= 0 ? hit.fraction : 1.0f;
-    for (int i = 0; i < n; i++) {
-        V3 a, b, c;
-        load_tri(w, cand_tri(cand[i]), a, b, c);
-        V3 nrm;
-        if (ray_triangle(from, to, a, b, c, frac, nrm)) {
-            hit.fraction = frac;
-            hit.normal = nrm;
-            hit.object = 0;
-        }
-    }
     for (int p = 0; p < kNumPlanes; p++) {
         // btStaticPlaneShape::processAllTriangles around the ray's local AABB,
         // then the triangle ray test on its two triangles.

@@ -82,6 +82,12 @@ class EnvConfig(C.Structure):
                 ("obs_builder", C.c_int32), ("no_touch_stagger", C.c_int32)]
 
 
+class VantaConfig(C.Structure):
+    """rsx_vanta_config: BatchConfig's vanta* fields (RocketSimCuda.h:184-189)."""
+    _fields_ = [("predict_ball", C.c_int32), ("only_closest_opp", C.c_int32), ("dodge_deadzone", C.c_float),
+                ("tick_skip", C.c_int32)]
+
+
 class TrainingRewardEntry(C.Structure):
     """rsx_training_reward_entry = TrainingRewardEntry (RocketSimCuda.h:235-242)."""
     _fields_ = [("id", C.c_uint8), ("weight", C.c_float), ("params", C.c_float * 10), ("is_zero_sum", C.c_uint8),
@@ -100,6 +106,8 @@ MAX_TRAINING_REWARDS = 16
 
 OBS_DEFAULT = 0
 OBS_ADVANCED = 1
+OBS_VANTA = 2
+VANTA_OBS_SIZE = 366
 
 
 CARS_BLUE_FIRST = 0

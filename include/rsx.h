@@ -137,7 +137,7 @@ typedef struct rsx_env_config {
     float no_touch_timeout_s;          /* NoTouchCondition(maxTime); <= 0 off; default rsx_config's */
     int32_t goal_terminal;             /* GoalScoreCondition on; default 1 */
     int32_t auto_reset;                /* kickoff reset of terminal arenas inside rsx_step_env; default 1 */
-    int32_t obs_builder;               /* RSX_OBS_DEFAULT (default) or RSX_OBS_ADVANCED */
+    int32_t obs_builder;               /* RSX_OBS_DEFAULT (default), RSX_OBS_ADVANCED or RSX_OBS_VANTA */
     int32_t no_touch_stagger;          /* a reset starts the no-touch timer at a uniform draw in
                                           [0, 0.9) x the timeout (ResetArena, RocketSimCudaApi.cuh:396-406);
                                           default 0: at 0, as NoTouchCondition::Reset */
@@ -148,10 +148,20 @@ typedef struct rsx_env_config {
  * action.  RSX_OBS_ADVANCED: the AdvancedObs rows of the reference's GPU
  * training path (BuildAdvancedObsAndDefaultMasks, RocketSimCuda.h:316;
  * TrainingObs.cuh:20-127), 51 + 29 * cars floats per car (109 / 167 / 225),
- * the previous action being the car's last controls.  Changing the builder
- * changes rsx_obs_layout and obs_floats (the obs buffer's address stays). */
+ * the previous action being the car's last controls.  RSX_OBS_VANTA: the
+ * VantaObs v3.5 rows of the reference's GPU path (ObsMode::VANTA,
+ * RocketSimCuda.h:168; BuildVantaObs, 318-321; VantaObsGpu.cuh:342-747),
+ * 366 floats per car whatever the team size, with per-arena pad timers,
+ * per-car coyote timers and optional ball-prediction slices (see
+ * rsx_configure_vanta).  rsx_step_env and rsx_build_obs then run the Vanta
+ * kernel after the step; an arena's Vanta state is cleared whenever it is
+ * reset (rsx_reset_arena, rsx_reset_all_arenas, the env step's auto-reset),
+ * as ResetArena clears it (RocketSimCudaApi.cuh:265-274).  Changing the
+ * builder changes rsx_obs_layout and obs_floats (the obs buffer's address
+ * stays). */
 #define RSX_OBS_DEFAULT 0
 #define RSX_OBS_ADVANCED 1
+#define RSX_OBS_VANTA 2
 
 /* ConfigureTrainingTerminals (RocketSimCuda.h:330) and the stub rewards. */
 rsx_status rsx_configure_env(rsx_engine* e, const rsx_env_config* c);
@@ -313,6 +323,26 @@ rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t);
  * Blocks the host until done. */
 rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t env_steps, rsx_tune_result* results,
                         int32_t capacity, int32_t* count);
+
+/* VantaObs parameters; replace BatchConfig's vantaPredictBall,
+ * vantaOnlyClosestOpp, vantaDodgeDeadzone and vantaTickSkip
+ * (RocketSimCuda.h:184-189).  Defaults: no prediction, every opponent,
+ * deadzone 0.5, the engine's tick skip. */
+typedef struct rsx_vanta_config {
+    int32_t predict_ball;      /* ball-prediction slices: the ball's position and velocity after 120, 240 and
+                                  360 ticks of a ball-only rollout with the engine's own physics
+                                  (vantaArenaUpdateKernel, RocketSimCudaObs.cuh:70-122) */
+    int32_t only_closest_opp;  /* fill only the closest opponent's packet */
+    float dodge_deadzone;      /* least |pitch|, |yaw| or |roll| a jump needs to count as a flip, in [0, 1] */
+    int32_t tick_skip;         /* the coyote / pad timers advance tick_skip / 120 s per update; 0: the engine's */
+} rsx_vanta_config;
+
+rsx_status rsx_configure_vanta(rsx_engine* e, const rsx_vanta_config* c);
+rsx_status rsx_get_vanta_config(const rsx_engine* e, rsx_vanta_config* c);
+/* The ball-prediction slices an arena's last Vanta build used (positions in
+ * uu, velocities in uu/s, angular velocities zero) and whether it had them;
+ * no counterpart in the reference (its slices live only on the device). */
+rsx_status rsx_get_vanta_prediction(rsx_engine* e, int32_t arena, rsx_ball_state* out3, int32_t* has_prediction);
 
 /* hipGraph of the env step: rsx_capture_env_graph records `env_steps`
  * consecutive rsx_step_env calls reading `device_actions` (the caller

@@ -5,6 +5,7 @@
 #include "../../emy-s-framework_amd/csrc/rsx_host.h"
 #include "../../emy-s-framework_amd/csrc/rsx_soa.h"
 #include "../../emy-s-framework_amd/csrc/rsx_train.h"
+#include "../../emy-s-framework_amd/csrc/rsx_vanta.h"
 #include "../../include/rsx.h"
 
 #include <cstring>
@@ -375,4 +376,85 @@ extern "C" int cc_curriculum_reset(void* w, int n_blue, int n_orange, uint32_t s
     ball_to_public(a->A.ball, *ball);
     cc_arena_free(a);
     return mode;
+}
+
+// ---- VantaObs (rsx_vanta.h) on the host ----------------------------------
+// A Vanta state is one arena's Vanta words (6 car slots, stride 1), all zero
+// when new or cleared.
+extern "C" void* cc_vanta_new() { return new std::vector<uint32_t>(vanta::VLayout<6>::kWords, 0u); }
+extern "C" void cc_vanta_free(void* v) { delete (std::vector<uint32_t>*)v; }
+extern "C" void cc_vanta_clear(void* v) { vanta::clear<6>(Col{((std::vector<uint32_t>*)v)->data(), 1}); }
+
+// The ball-only rollout of the arena's ball: 3 slices of (pos, vel), uu.
+extern "C" void cc_vanta_predict(void* p, float* out18) {
+    auto* a = (CcArena*)p;
+    TickCtx T;
+    T.cg = a->world->w.cg;
+    T.bg = a->world->w.bg;
+    T.mv = a->world->mv;
+    T.pads = a->world->w.pads;
+    T.dt = 1.f / kTickRate;
+    gjk::Simplex sx;
+    T.sx = &sx;
+    T.cull = sdf::Cull{};
+    T.lds_act = nullptr;
+    T.own = true;
+    T.pool = nullptr;
+    auto* W = new TickScratch<2>();
+    W->cull_age = 1 << 30;
+    W->L.overflow = false;
+    auto* P = new Arena<2>();
+    V3 pos[3], vel[3];
+    vanta::predict_ball(a->A.ball, T, *W, *P, pos, vel);
+    for (int k = 0; k < 3; k++) {
+        out18[6 * k + 0] = pos[k].x;
+        out18[6 * k + 1] = pos[k].y;
+        out18[6 * k + 2] = pos[k].z;
+        out18[6 * k + 3] = vel[k].x;
+        out18[6 * k + 4] = vel[k].y;
+        out18[6 * k + 5] = vel[k].z;
+    }
+    delete P;
+    delete W;
+}
+
+// BuildVantaObs for the arena with its Vanta state: pred18 (null: no
+// prediction) holds the slices; rows of every car to obs (ncars x 366).
+extern "C" void cc_vanta_build(void* p, void* v, const float* pred18, float dt, float deadzone, int only_closest,
+                               float* obs) {
+    auto* a = (CcArena*)p;
+    const Col c{((std::vector<uint32_t>*)v)->data(), 1};
+    EnvConfig E;
+    obs_pad_map(a->world->w.pads, E.pad_map);
+    vanta::PadTables P{};
+    for (int i = 0; i < vanta::kPads; i++) {
+        P.map[i] = E.pad_map[i];
+        P.loc[i] = a->world->w.pads[E.pad_map[i]].pos_uu;
+    }
+    for (int i = 0; i < vanta::kPads; i++) {
+        int m = i;
+        for (int j = 0; j < vanta::kPads; j++)
+            if (fabsf(P.loc[j].x + P.loc[i].x) < 1.f && fabsf(P.loc[j].y - P.loc[i].y) < 1.f &&
+                fabsf(P.loc[j].z - P.loc[i].z) < 1.f) {
+                m = j;
+                break;
+            }
+        P.mirror[i] = (int8_t)m;
+    }
+    vanta::Params Q{dt, deadzone, only_closest, pred18 != nullptr};
+    vanta::State<6> S;
+    vanta::load<6>(c, S);
+    if (pred18) {
+        for (int k = 0; k < 3; k++) {
+            S.pred_pos[k] = V3{pred18[6 * k], pred18[6 * k + 1], pred18[6 * k + 2]};
+            S.pred_vel[k] = V3{pred18[6 * k + 3], pred18[6 * k + 4], pred18[6 * k + 5]};
+        }
+        S.has_pred = true;
+    } else {
+        S.has_pred = false;
+    }
+    vanta::update_arena<6>(S, a->A, P, dt);
+    const uint32_t lt = vanta::last_touch_id<6>(a->A);
+    for (int i = 0; i < a->A.ncars; i++) vanta::build_row<6>(a->A, S, P, Q, i, lt, obs + (int64_t)i * vanta::kRow);
+    vanta::store<6>(c, S, a->A.ncars);
 }

@@ -151,6 +151,12 @@ def load_cpucheck():
                                     C.c_uint64, C.c_int, C.POINTER(C.c_float)]),
         ("cc_curriculum_reset", C.c_int, [VP, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_float),
                                           C.POINTER(CarState), C.POINTER(BallState)]),
+        ("cc_vanta_new", VP, []),
+        ("cc_vanta_free", None, [VP]),
+        ("cc_vanta_clear", None, [VP]),
+        ("cc_vanta_predict", None, [VP, C.POINTER(C.c_float)]),
+        ("cc_vanta_build", None, [VP, VP, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
+                                  C.POINTER(C.c_float)]),
     ])
 
 
@@ -162,6 +168,18 @@ def load_spec():
     spec = importlib.util.spec_from_file_location("rlgym_spec", path)
     mod = importlib.util.module_from_spec(spec)
     sys.modules["rlgym_spec"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_vanta_spec():
+    """The CPU restatement of the VantaObs rows (oracle/vanta_spec.py)."""
+    if "vanta_spec" in sys.modules:
+        return sys.modules["vanta_spec"]
+    path = os.path.join(ROOT, "oracle", "vanta_spec.py")
+    spec = importlib.util.spec_from_file_location("vanta_spec", path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["vanta_spec"] = mod
     spec.loader.exec_module(mod)
     return mod
 

@@ -41,14 +41,15 @@ def test_exports_every_declared_symbol():
 def test_record_layouts_match_header(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "rsx.h"\nint main(void){printf("%zu %zu %zu '
-                   '%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(rsx_car_state), sizeof(rsx_ball_state), '
+                   '%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(rsx_car_state), sizeof(rsx_ball_state), '
                    'sizeof(rsx_pad_state), sizeof(rsx_controls), sizeof(rsx_arena_info), '
                    'offsetof(rsx_car_state, last_controls), offsetof(rsx_car_state, ball_hit_tick), '
                    'offsetof(rsx_arena_info, goal_team), offsetof(rsx_arena_info, contact_overflow), '
                    'sizeof(rsx_env_config), offsetof(rsx_env_config, obs_builder), '
                    'offsetof(rsx_env_config, no_touch_stagger), sizeof(rsx_training_reward_entry), '
                    'offsetof(rsx_training_reward_entry, is_zero_sum), '
-                   'offsetof(rsx_training_reward_entry, opponent_scale));return 0;}\n')
+                   'offsetof(rsx_training_reward_entry, opponent_scale), sizeof(rsx_vanta_config), '
+                   'offsetof(rsx_vanta_config, tick_skip));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
@@ -57,11 +58,12 @@ def test_record_layouts_match_header(tmp_path):
             C.sizeof(E.ArenaInfo), E.CarState.last_controls.offset, E.CarState.ball_hit_tick.offset,
             E.ArenaInfo.goal_team.offset, E.ArenaInfo.contact_overflow.offset, C.sizeof(E.EnvConfig),
             E.EnvConfig.obs_builder.offset, E.EnvConfig.no_touch_stagger.offset, C.sizeof(E.TrainingRewardEntry),
-            E.TrainingRewardEntry.is_zero_sum.offset, E.TrainingRewardEntry.opponent_scale.offset]
+            E.TrainingRewardEntry.is_zero_sum.offset, E.TrainingRewardEntry.opponent_scale.offset,
+            C.sizeof(E.VantaConfig), E.VantaConfig.tick_skip.offset]
     assert got == want
     # TrainingRewardEntry's own layout (RocketSimCuda.h:235-242: uint8 id, float weight,
     # float params[10], uint8 isZeroSum, float teamSpirit, float opponentScale)
-    assert got[-3:] == [60, 48, 56]
+    assert got[-5:-2] == [60, 48, 56]
 
 
 def test_invalid_config_is_rejected():
