@@ -151,7 +151,7 @@ RSX_HD void manifold_refresh(Manifold& m, const Xf& trA, const Xf& trB) {
 template <int MAXC>
 struct MfList {
     static constexpr int kCap = (MAXC + 1) * (kNumMeshes + kNumPlanes) + (MAXC + 1) * MAXC / 2;
-    static constexpr int kLive = 12 + 2 * (MAXC - 2);
+    static constexpr int kLive = MAXC >= 2 ? 12 + 2 * (MAXC - 2) : 12;
     int count;
     int8_t body0[kCap], body1[kCap];  // 0 ball, 1+i car i, -1 static
     int8_t slot[kCap];                // index into live, -1 empty

@@ -5,6 +5,8 @@
 // boost pads, ball limits and the goal test.
 #pragma once
 
+#include <type_traits>
+
 #include "rsx_boxbox.h"
 #include "rsx_gjk.h"
 #include "rsx_solver.h"
@@ -52,6 +54,17 @@ RSX_HD int uf_find(int* id, int x) {
     return x;
 }
 
+// A ball-only tick has no box queries, so it keeps no GJK / EPA working set.
+struct NoGjk {};
+
+// Cars in use.  An arena of MAXC = 0 car slots is the ball alone (the
+// ball-prediction rollouts), for which the count is a compile-time zero and
+// every car branch of the tick drops out.
+template <int MAXC>
+RSX_HD int ncars_of(const Arena<MAXC>& A) {
+    return MAXC == 0 ? 0 : A.ncars;
+}
+
 template <int MAXC>
 struct TickScratch {
     MfList<MAXC> L;
@@ -60,7 +73,7 @@ struct TickScratch {
     int cand[kMaxCand];
     int16_t pidx[kMaxCand];              // candidate -> index among the lane's pooled queries, -1: separated
     int pool_base;                       // the lane's first slot in the wavefront's pooled results
-    gjk::GjkEpa gj;                      // box-vs-triangle GJK / EPA working set
+    std::conditional_t<(MAXC > 0), gjk::GjkEpa, NoGjk> gj;  // box-vs-triangle GJK / EPA working set
     float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
     int cull_age;                        // ticks since the network last ran (uniform over the wavefront)
     unsigned long long cull_counts[3];   // the wavefront's queries, culled body-ticks, audit defects
@@ -244,6 +257,7 @@ constexpr int kPoolBytes = kPoolResOff + kPoolCap * (int)sizeof(PooledContact);
 template <int MAXC>
 __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int ci, uint32_t mask,
                              int ncand) {
+  if constexpr (MAXC > 0) {
     const int l = threadIdx.x & 63;
     const CarGeom& g = T.cg;
     Xf* xfs = (Xf*)(T.pool + kPoolXfOff);
@@ -301,6 +315,7 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
         }
     }
     sdf::wave_sync();
+  }
 }
 #endif
 
@@ -313,6 +328,7 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
 template <int MAXC>
 RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
                         int ncand) {
+  if constexpr (MAXC > 0) {
     gjk::GjkEpa& G = W.gj;
     const int* cand = W.cand;
     const CarGeom& g = T.cg;
@@ -354,6 +370,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
         }
     }
     if (m.n) manifold_refresh(m, c.xf, trB);
+  }
 }
 
 // GJK result for the ball's centre point against the car's core box (margins
@@ -394,12 +411,12 @@ RSX_HD bool sphere_box(V3 c, float r, const Xf& box, V3 h, float margin, float b
 template <int MAXC>
 RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, const bool* far) {
     const float dt = T.dt;
-    const int nb = 1 + A.ncars;
+    const int nb = 1 + ncars_of(A);
     const CarGeom& g = T.cg;
     // gravity on active bodies (applyGravity): m_gravity = acc * (1 / invMass)
     const V3 gacc = V3{0, 0, kGravityZ} * kUU2BT;
     if (!A.ball.sleeping) A.ball.force += gacc * (1.0f / T.bg.inv_mass);
-    for (int i = 0; i < A.ncars; i++)
+    for (int i = 0; i < ncars_of(A); i++)
         if (A.car[i].active) A.car[i].force += gacc * (1.0f / g.inv_mass);
 
     // predictUnconstraintMotion: damping + predicted transforms
@@ -409,7 +426,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         b.ang *= powf(1.f - 0.f, dt);
         b.interp = Xf{b.xf.b, b.xf.o + b.vel * dt};
     }
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         Car& c = A.car[i];
         c.vel *= powf(1.f - 0.f, dt);
         c.ang *= powf(1.f - 0.f, dt);
@@ -422,14 +439,14 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         ball_shape_aabb(T.bg, A.ball.xf.o, a0, a1);
         ball_shape_aabb(T.bg, A.ball.interp.o, b0, b1);
         broad_aabb(a0, a1, b0, b1, W.bmin[0], W.bmax[0]);
-        for (int i = 0; i < A.ncars; i++) {
+        for (int i = 0; i < ncars_of(A); i++) {
             car_shape_aabb(g, A.car[i].xf, a0, a1);
             car_shape_aabb(g, A.car[i].interp, b0, b1);
             broad_aabb(a0, a1, b0, b1, W.bmin[1 + i], W.bmax[1 + i]);
         }
         // btRSBroadphase::setAabb: the registrations follow the new AABBs
         A.ball.bp_reg = bp_cell_index(T.mv, W.bmin[0]);
-        for (int i = 0; i < A.ncars; i++) A.car[i].bp_reg = bp_cell_index(T.mv, W.bmin[1 + i]);
+        for (int i = 0; i < ncars_of(A); i++) A.car[i].bp_reg = bp_cell_index(T.mv, W.bmin[1 + i]);
     }
 
     // ---- pairs, manifolds, contacts (dispatch order) ----
@@ -440,7 +457,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     for (int i = 0; i < nb; i++) uf[i] = i;
     bool active[MAXC + 1];
     active[0] = !A.ball.sleeping;
-    for (int i = 0; i < A.ncars; i++) active[1 + i] = A.car[i].active;
+    for (int i = 0; i < ncars_of(A); i++) active[1 + i] = A.car[i].active;
 
     for (int d = 0; d < nb; d++) {
         // a car demolished at the start of the tick has simulation and contact
@@ -665,7 +682,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         M3 ii = m3(T.bg.inv_inertia, 0, 0, 0, T.bg.inv_inertia, 0, 0, 0, T.bg.inv_inertia);
         sbody_init(S.body[0], b.xf, b.vel, b.ang, b.force, V3{0, 0, 0}, T.bg.inv_mass, ii, dt);
     }
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         if (!in_solver[1 + i]) continue;
         Car& c = A.car[i];
         sbody_init(S.body[1 + i], c.xf, c.vel, c.ang, c.force, c.torque, g.inv_mass, c.inv_inertia_world, dt);
@@ -729,7 +746,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         Ball& b = A.ball;
         sbody_writeback(S.body[0], true, dt, b.xf, b.vel, b.ang);
     }
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         if (!in_solver[1 + i]) continue;
         Car& c = A.car[i];
         sbody_writeback(S.body[1 + i], false, dt, c.xf, c.vel, c.ang);
@@ -740,7 +757,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         Ball& b = A.ball;
         b.xf.o = b.xf.o + b.vel * dt;
     }
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         Car& c = A.car[i];
         if (!c.active) continue;
         c.xf = integrate_transform(c.xf, c.vel, c.ang, dt);
@@ -748,7 +765,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     }
     // clearForces
     A.ball.force = V3{0, 0, 0};
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         A.car[i].force = V3{0, 0, 0};
         A.car[i].torque = V3{0, 0, 0};
     }
@@ -788,7 +805,7 @@ RSX_HD void pad_check_car(Arena<MAXC>& A, const TickCtx& T, int ci) {
 
 template <int MAXC>
 RSX_HD Car* car_by_id(Arena<MAXC>& A, uint32_t id) {
-    for (int i = 0; i < A.ncars; i++)
+    for (int i = 0; i < ncars_of(A); i++)
         if (A.car[i].id == id) return &A.car[i];
     return nullptr;
 }
@@ -847,7 +864,7 @@ __device__ void cull_far(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAX
     if (eval) W.cull_age = 0;
     unsigned long long nq = 0, nc = 0, nd = 0;
     for (int b = 0; b <= MAXC; b++) {
-        const bool have = (b == 0) || (b - 1 < A.ncars);
+        const bool have = (b == 0) || (b - 1 < ncars_of(A));
         V3 q = A.ball.xf.o;
         if (b > 0 && have) q = car_child_xf(T.cg, A.car[b - 1].xf).o;
         if (eval) {
@@ -911,7 +928,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     // order decides which car's wheel rays see another car's response flag
     // as of this tick, whose impulses a car on another car reads, and which
     // of two cars on one pad locks it.
-    for (int i = A.ncars - 1; i >= 0; i--) {
+    for (int i = ncars_of(A) - 1; i >= 0; i--) {
         Car& c = A.car[i];
         Controls& k = c.controls;
         k.throttle = clampf(k.throttle, -1, 1);
@@ -932,7 +949,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
         if (c.is_demoed) continue;
         car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i]);
     }
-    if (A.ncars > 0) {
+    if (ncars_of(A) > 0) {
         for (int p = 0; p < kNumPads; p++) {
             Pad& pad = A.pad[p];
             if (pad.cooldown > 0) pad.cooldown = maxf(pad.cooldown - dt, 0);
@@ -940,10 +957,10 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
             pad.cur_locked = 0;
         }
     }
-    M3 rot0[MAXC];
-    for (int i = 0; i < A.ncars; i++) rot0[i] = A.car[i].xf.b;
+    M3 rot0[MAXC > 0 ? MAXC : 1];
+    for (int i = 0; i < ncars_of(A); i++) rot0[i] = A.car[i].xf.b;
     arena_world_step(A, T, W, far);
-    for (int i = A.ncars - 1; i >= 0; i--) {  // post-tick and pads, last-added car first
+    for (int i = ncars_of(A) - 1; i >= 0; i--) {  // post-tick and pads, last-added car first
         Car& c = A.car[i];
         // A car demolished during this step skips Car::_PostTickUpdate, which
         // is where its public rotation would have followed the body; the body
@@ -954,7 +971,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
         car_finish_tick(c);
         pad_check_car(A, T, i);
     }
-    if (A.ncars > 0) {
+    if (ncars_of(A) > 0) {
         for (int p = 0; p < kNumPads; p++) {
             Pad& pad = A.pad[p];
             uint32_t locked = 0;
@@ -1061,7 +1078,7 @@ RSX_HD void arena_reset_kickoff(Arena<MAXC>& A, const CarGeom& g) {
         order[j] = t;
     }
     int nb = 0, no = 0;
-    for (int i = 0; i < A.ncars; i++) {
+    for (int i = 0; i < ncars_of(A); i++) {
         Car& c = A.car[i];
         int slot = (c.team == 0) ? nb++ : no++;
         float x, y, yaw;

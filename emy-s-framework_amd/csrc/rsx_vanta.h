@@ -695,9 +695,10 @@ RSX_HD void build_row(const Arena<MAXC>& A, State<MAXC>& s, const PadTables& P, 
 }
 
 // Ball-only rollout of `ball` (engine units): its position (uu) and
-// velocity (uu/s) after 120, 240 and 360 ticks.  W is the tick's working
-// set; T the tick context (no cull, no pooling: there are no cars).
-RSX_HD void predict_ball(const Ball& ball, const TickCtx& T, TickScratch<2>& W, Arena<2>& P, V3* pos, V3* vel) {
+// velocity (uu/s) after 120, 240 and 360 ticks.  P is an arena without car
+// slots, so the tick compiles to the ball's own path; W is its working set,
+// T the tick context (no cull, no pooling: there are no cars).
+RSX_HD void predict_ball(const Ball& ball, const TickCtx& T, TickScratch<0>& W, Arena<0>& P, V3* pos, V3* vel) {
     P.ncars = 0;
     P.tick = 0;
     P.rng = 0;
@@ -708,7 +709,7 @@ RSX_HD void predict_ball(const Ball& ball, const TickCtx& T, TickScratch<2>& W, 
     P.ball.sleeping = false;
     int slice = 0;
     for (int t = 1; t <= kPredTicks; t++) {
-        (void)arena_tick<2>(P, T, W);
+        (void)arena_tick<0>(P, T, W);
         if (t == 120 || t == 240 || t == 360) {
             pos[slice] = P.ball.xf.o * kBT2UU;
             vel[slice] = P.ball.vel * kBT2UU;

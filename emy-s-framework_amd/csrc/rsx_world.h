@@ -174,7 +174,7 @@ template <int MAXC>
 struct Arena {
     int ncars;
     uint64_t tick;
-    Car car[MAXC];
+    Car car[MAXC > 0 ? MAXC : 1];  // MAXC = 0: the ball alone (one unused slot)
     Ball ball;
     Pad pad[kNumPads];
     uint32_t rng;              // per-arena RNG state (kickoff / demo respawn draws)

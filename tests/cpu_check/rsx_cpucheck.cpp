@@ -400,10 +400,10 @@ extern "C" void cc_vanta_predict(void* p, float* out18) {
     T.lds_act = nullptr;
     T.own = true;
     T.pool = nullptr;
-    auto* W = new TickScratch<2>();
+    auto* W = new TickScratch<0>();
     W->cull_age = 1 << 30;
     W->L.overflow = false;
-    auto* P = new Arena<2>();
+    auto* P = new Arena<0>();
     V3 pos[3], vel[3];
     vanta::predict_ball(a->A.ball, T, *W, *P, pos, vel);
     for (int k = 0; k < 3; k++) {

@@ -443,3 +443,24 @@ def test_cpu_build_ball_prediction_against_the_oracle(oracle, cpucheck):
         want = _oracle_prediction(oracle, b)
         assert np.abs(got[:, :3] - want[:, :3]).max() <= 1e-4 * 2300, (k, got, want)
         assert np.abs(got[:, 3:] - want[:, 3:]).max() <= 1e-4 * 6000, (k, got, want)
+
+
+def test_ball_only_tick_equals_the_arena_tick_without_cars(cpucheck):
+    """The prediction's tick (an arena with no car slots, whose car branches
+    compile away) gives the same ball, bit for bit, as the full arena tick
+    stepping an arena that holds no cars."""
+    lib, w = cpucheck
+    rng = np.random.default_rng(12)
+    pred = (C.c_float * 18)()
+    for b in _prediction_states(rng, 45):
+        h = C.c_void_p(lib.cc_arena_new(w, 0, 0))
+        lib.cc_set_ball(h, C.byref(b))
+        lib.cc_vanta_predict(h, pred)
+        full = []
+        for _ in range(3):
+            lib.cc_step(h, 120)
+            s = R.BallState()
+            lib.cc_get_ball(h, C.byref(s))
+            full.append([s.pos.x, s.pos.y, s.pos.z, s.vel.x, s.vel.y, s.vel.z])
+        lib.cc_arena_free(h)
+        np.testing.assert_array_equal(np.array(pred, dtype=F).reshape(3, 6), np.array(full, dtype=F))
