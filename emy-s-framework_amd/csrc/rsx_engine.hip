@@ -47,6 +47,8 @@ struct DevWorld {
     int* cell_start = nullptr;
     int* cell_tris = nullptr;
     GridEntry* cell_ent = nullptr;
+    int* blk_start = nullptr;
+    GridEntry* blk_ent = nullptr;
     uint16_t* bp_meshes = nullptr;
     float* mesh_aabb = nullptr;
     float* tri_info = nullptr;
@@ -66,6 +68,7 @@ struct Bucket {
     int* d_global = nullptr;         // n: global arena index
     std::vector<int> global;         // bucket-local -> global arena index
     uint32_t* d_vanta = nullptr;     // VantaObs state words (field-major), allocated with the first Vanta config
+    int* d_vwork = nullptr;          // n + 1 ints: the rollouts' arena list and its length
 };
 
 struct rsx_engine {
@@ -104,7 +107,7 @@ struct rsx_engine {
     int rt_n = 0;
     std::vector<rsx_training_reward_entry> rt_host;
     // VantaObs (rsx_configure_vanta): parameters, pad tables, state allocated
-    vanta::Params vq{8.f / 120.f, 0.5f, 0, 0};
+    vanta::Params vq{8.f / 120.f, 0.5f, 0, 0, 8};
     vanta::PadTables vpads{};
     bool vanta_alloc = false;
     // captured env steps (rsx_capture_env_graph)
@@ -318,7 +321,7 @@ hipError_t vanta_launch(rsx_engine* e, const uint8_t* reset, hipStream_t st) {
     v.obs = e->d_obs;
     for (const Bucket& b : e->bucket) {
         if (!b.n) continue;
-        const hipError_t r = launch_vanta(b.maxc, bucket_args(b), b.d_vanta, reset, v, st);
+        const hipError_t r = launch_vanta(b.maxc, bucket_args(b), b.d_vanta, reset, v, b.d_vwork, b.d_vwork + b.n, st);
         if (r != hipSuccess) return r;
     }
     return hipSuccess;
@@ -366,6 +369,8 @@ rsx_status vanta_alloc(rsx_engine* e) {
         const size_t bytes = sizeof(uint32_t) * (size_t)vanta_words(b.maxc) * b.n;
         HIP_TRY(hipMalloc((void**)&b.d_vanta, bytes));
         HIP_TRY(hipMemset(b.d_vanta, 0, bytes));
+        HIP_TRY(hipMalloc((void**)&b.d_vwork, sizeof(int) * ((size_t)b.n + 1)));
+        HIP_TRY(hipMemset(b.d_vwork, 0, sizeof(int) * ((size_t)b.n + 1)));
     }
     e->vanta_alloc = true;
     return RSX_OK;
@@ -449,6 +454,8 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     if ((s = upload(&e->dw.cell_start, e->hw.cell_start)) != RSX_OK) return s;
     if ((s = upload(&e->dw.cell_tris, e->hw.cell_tris)) != RSX_OK) return s;
     if ((s = upload(&e->dw.cell_ent, e->hw.cell_ent)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.blk_start, e->hw.blk_start)) != RSX_OK) return s;
+    if ((s = upload(&e->dw.blk_ent, e->hw.blk_ent)) != RSX_OK) return s;
     if ((s = upload(&e->dw.bp_meshes, e->hw.bp_meshes)) != RSX_OK) return s;
     if ((s = upload(&e->dw.mesh_aabb, e->hw.mesh_aabb)) != RSX_OK) return s;
     if ((s = upload(&e->dw.tri_info, e->hw.tri_info)) != RSX_OK) return s;
@@ -461,6 +468,8 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
     e->dview.cell_start = e->dw.cell_start;
     e->dview.cell_tris = e->dw.cell_tris;
     e->dview.cell_ent = e->dw.cell_ent;
+    e->dview.blk_start = e->dw.blk_start;
+    e->dview.blk_ent = e->dw.blk_ent;
     e->dview.bp_meshes = e->dw.bp_meshes;
     e->dview.mesh_aabb = e->dw.mesh_aabb;
     e->dview.tri_info = e->dw.tri_info;
@@ -522,6 +531,7 @@ rsx_status rsx_create(const rsx_config* cfg, rsx_engine** out) {
         if ((s = upload(&e->d_mask_tables, default_action_masks())) != RSX_OK) return s;
         e->vpads = vanta_pad_tables(e->hw.pads, E.pad_map);
         e->vq.dt = (float)e->tick_skip / 120.f;
+        e->vq.ticks_between = e->tick_skip;
     }
     for (int k = 0; k < 3; k++) {
         Bucket& b = e->bucket[k];
@@ -572,6 +582,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
         (void)hipFree(b.d_obs_off);
         (void)hipFree(b.d_global);
         (void)hipFree(b.d_vanta);
+        (void)hipFree(b.d_vwork);
     }
     (void)hipFree(e->d_obs);
     (void)hipFree(e->d_rew);
@@ -582,6 +593,7 @@ rsx_status rsx_destroy(rsx_engine* e) {
     (void)hipFree(e->d_idx_stage);
     (void)hipFree(e->d_rt);
     void* ptrs[] = {e->dw.tri, e->dw.tri_mesh, e->dw.tri_rank, e->dw.cell_start, e->dw.cell_tris, e->dw.cell_ent,
+                    e->dw.blk_start, e->dw.blk_ent,
                     e->dw.bp_meshes, e->dw.mesh_aabb, e->dw.tri_info, e->dw.pads};
     for (void* p : ptrs) (void)hipFree(p);
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);

@@ -27,6 +27,8 @@ MeshView HostWorld::view() const {
     v.cell_start = cell_start.data();
     v.cell_tris = cell_tris.data();
     v.cell_ent = cell_ent.data();
+    v.blk_start = blk_start.data();
+    v.blk_ent = blk_ent.data();
     v.susp_world = nullptr;
     v.bp_meshes = bp_meshes.data();
     v.mesh_aabb = mesh_aabb.data();
@@ -452,6 +454,40 @@ bool build_world(const std::vector<std::vector<uint8_t>>& files, HostWorld& w, s
             g.mn[0] = mn.x, g.mn[1] = mn.y, g.mn[2] = mn.z;
             g.mx[0] = mx.x, g.mx[1] = mx.y, g.mx[2] = mx.z;
         }
+    }
+
+    // Block lists: for every cell, the distinct triangles of the 2 x 2 x 2
+    // block of cells whose lowest corner it is (clamped at the grid's far
+    // faces), sorted by packed key.  A body query box spans at most two cells
+    // per axis, so the block at its lowest cell holds every triangle a walk
+    // over its cells would find, once each and already in key order.
+    {
+        std::vector<GridEntry> ent_of(ntri);
+        for (size_t k = 0; k < w.cell_tris.size(); k++) ent_of[w.cell_tris[k]] = w.cell_ent[k];
+        std::vector<int> mark(ntri, -1), list;
+        w.blk_start.assign(ncell + 1, 0);
+        w.blk_ent.clear();
+        for (int x = 0; x < w.gx; x++)
+            for (int y = 0; y < w.gy; y++)
+                for (int z = 0; z < w.gz; z++) {
+                    const int blk = (x * w.gy + y) * w.gz + z;
+                    w.blk_start[blk] = (int)w.blk_ent.size();
+                    list.clear();
+                    for (int ix = x; ix <= std::min(x + 1, w.gx - 1); ix++)
+                        for (int iy = y; iy <= std::min(y + 1, w.gy - 1); iy++)
+                            for (int iz = z; iz <= std::min(z + 1, w.gz - 1); iz++) {
+                                const int c = (ix * w.gy + iy) * w.gz + iz;
+                                for (int k = w.cell_start[c]; k < w.cell_start[c + 1]; k++)
+                                    if (mark[w.cell_tris[k]] != blk) {
+                                        mark[w.cell_tris[k]] = blk;
+                                        list.push_back(w.cell_tris[k]);
+                                    }
+                            }
+                    std::sort(list.begin(), list.end(),
+                              [&](int a, int b) { return ent_of[a].key < ent_of[b].key; });
+                    for (int t : list) w.blk_ent.push_back(ent_of[t]);
+                }
+        w.blk_start[ncell] = (int)w.blk_ent.size();
     }
 
     // RocketSim broadphase static cell lists (btRSBroadphase.cpp:72-136): a

@@ -17,6 +17,8 @@ struct HostWorld {
     std::vector<int> tri_mesh, tri_rank;
     std::vector<int> cell_start, cell_tris;
     std::vector<GridEntry> cell_ent;
+    std::vector<int> blk_start;
+    std::vector<GridEntry> blk_ent;
     std::vector<uint16_t> bp_meshes;
     std::vector<float> mesh_aabb;  // 16 * 6
     std::vector<float> tri_info;   // num_tris * 4

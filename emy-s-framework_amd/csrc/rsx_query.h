@@ -83,6 +83,21 @@ RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
     int y1 = clampi((int)floorf((mx.y - w.grid_min.y) * inv), 0, w.gy - 1);
     int z1 = clampi((int)floorf((mx.z - w.grid_min.z) * inv), 0, w.gz - 1);
     int n = 0;
+    if (x1 - x0 <= 1 && y1 - y0 <= 1 && z1 - z0 <= 1) {
+        // the box's cells lie in the block at its lowest cell: a triangle
+        // whose AABB meets the box is listed in one of them (same cell
+        // arithmetic), the block lists it once, in key order
+        const int blk = (x0 * w.gy + y0) * w.gz + z0;
+        const int s = w.blk_start[blk], e = w.blk_start[blk + 1];
+        for (int k = s; k < e; k++) {
+            const GridEntry g = w.blk_ent[k];
+            if (g.mn[2] > mx.z || g.mx[2] < mn.z) continue;
+            if (g.mn[0] > mx.x || g.mx[0] < mn.x) continue;
+            if (g.mn[1] > mx.y || g.mx[1] < mn.y) continue;
+            if (n < cap) out[n++] = g.key;
+        }
+        return n;
+    }
     for (int ix = x0; ix <= x1; ix++)
         for (int iy = y0; iy <= y1; iy++)
             for (int iz = z0; iz <= z1; iz++) {

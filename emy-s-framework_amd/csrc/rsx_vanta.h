@@ -44,7 +44,15 @@ constexpr int kVaAvailHi = 35;      // pads 32-33 in bits 0-1; bit 8 initialized
 constexpr int kVaTick = 36;         // 2 words; ~0: never updated
 constexpr int kVaPredPos = 38;      // 3 x 3 floats (uu)
 constexpr int kVaPredVel = 47;      // 3 x 3 floats (uu/s)
-constexpr int kVaWords = 56;
+// The last rollout's continuation (bit 10 of kVaAvailHi: present): the arena
+// tick it started at, the ball it started from and the ball states it passed
+// at ts, 120 + ts, 240 + ts and 360 ticks (ts: physics ticks between builds),
+// each as position, velocity, angular velocity in engine units.
+constexpr int kVaT0 = 56;           // 2 words
+constexpr int kVaStart = 58;        // 9 floats
+constexpr int kVaChain = 67;        // 4 x 9 floats
+constexpr int kVaWords = 103;
+constexpr uint32_t kVaChainsBit = 1u << 10;
 // per-car words
 constexpr int kVcTimers = 0;        // boost, boost hold, jump, flip, air, handbrake, demo
 constexpr int kVcFlags = 7;         // bits: jumping, jumped, double jumped, flipped, on ground, was demoed,
@@ -93,6 +101,7 @@ struct Params {
     float deadzone;        // dodge deadzone of the flip detection
     int only_closest_opp;  // only the closest opponent's packet is filled
     int predict;           // ball-prediction slices on
+    int ticks_between;     // physics ticks between two env steps (the rollout continuation's step)
 };
 
 template <int MAXC>
@@ -143,7 +152,8 @@ template <int MAXC>
 RSX_HD void store(const Col& c, const State<MAXC>& s, int ncars) {
     for (int i = 0; i < kPads; i++) c.setf(kVaTimers + i, s.timers[i]);
     c.set(kVaAvail, (uint32_t)s.avail);
-    c.set(kVaAvailHi, (uint32_t)((s.avail >> 32) & 3u) | ((uint32_t)s.init << 8) | ((uint32_t)s.has_pred << 9));
+    c.set(kVaAvailHi, (uint32_t)((s.avail >> 32) & 3u) | ((uint32_t)s.init << 8) | ((uint32_t)s.has_pred << 9) |
+                          (c.get(kVaAvailHi) & kVaChainsBit));
     c.set64(kVaTick, s.last_tick);
     for (int k = 0; k < 3; k++) {
         c.setv(kVaPredPos + 3 * k, s.pred_pos[k]);
@@ -694,28 +704,103 @@ RSX_HD void build_row(const Arena<MAXC>& A, State<MAXC>& s, const PadTables& P, 
     }
 }
 
-// Ball-only rollout of `ball` (engine units): its position (uu) and
-// velocity (uu/s) after 120, 240 and 360 ticks.  P is an arena without car
-// slots, so the tick compiles to the ball's own path; W is its working set,
-// T the tick context (no cull, no pooling: there are no cars).
-RSX_HD void predict_ball(const Ball& ball, const TickCtx& T, TickScratch<0>& W, Arena<0>& P, V3* pos, V3* vel) {
-    P.ncars = 0;
-    P.tick = 0;
-    P.rng = 0;
-    P.ball = ball;
-    P.ball.xf.b = m3_identity();
-    P.ball.force = V3{0, 0, 0};
-    P.ball.vel_cache = V3{0, 0, 0};
-    P.ball.sleeping = false;
+// A ball of the ball-only rollout: position, velocity, angular velocity
+// (engine units).  The ball-only tick is a function of these nine floats
+// alone (the basis stays identity, forces and impulse caches are zero
+// between ticks, the sleep state and the contact manifolds are rebuilt every
+// tick), so two rollouts from equal balls pass through equal balls.
+struct BallQ {
+    V3 p, v, w;
+};
+
+RSX_HD BallQ ball_q(const Ball& b) { return BallQ{b.xf.o, b.vel, b.ang}; }
+
+RSX_HD bool same_ball(const BallQ& a, const BallQ& b) {
+    return f2u(a.p.x) == f2u(b.p.x) && f2u(a.p.y) == f2u(b.p.y) && f2u(a.p.z) == f2u(b.p.z) &&
+           f2u(a.v.x) == f2u(b.v.x) && f2u(a.v.y) == f2u(b.v.y) && f2u(a.v.z) == f2u(b.v.z) &&
+           f2u(a.w.x) == f2u(b.w.x) && f2u(a.w.y) == f2u(b.w.y) && f2u(a.w.z) == f2u(b.w.z);
+}
+
+RSX_HD BallQ get_q(const Col& c, int w) { return BallQ{c.getv(w), c.getv(w + 3), c.getv(w + 6)}; }
+RSX_HD void set_q(const Col& c, int w, const BallQ& q) {
+    c.setv(w, q.p);
+    c.setv(w + 3, q.v);
+    c.setv(w + 6, q.w);
+}
+
+// The ball-only arena's working state: P has no car slots, so the tick
+// compiles to the ball's own path; W is its working set, T the tick context
+// (no cull, no pooling: there are no cars).
+struct Roller {
+    const TickCtx& T;
+    TickScratch<0>& W;
+    Arena<0>& P;
+    RSX_HD void load(const BallQ& q) {
+        P.ncars = 0;
+        P.tick = 0;
+        P.rng = 0;
+        P.ball.xf = Xf{m3_identity(), q.p};
+        P.ball.interp = P.ball.xf;
+        P.ball.vel = q.v;
+        P.ball.ang = q.w;
+        P.ball.force = V3{0, 0, 0};
+        P.ball.vel_cache = V3{0, 0, 0};
+        P.ball.sleeping = false;
+        P.ball.bp_reg = -1;
+    }
+    RSX_HD void tick() { (void)arena_tick<0>(P, T, W); }
+    RSX_HD BallQ q() const { return ball_q(P.ball); }
+    RSX_HD BallQ advance(const BallQ& q, int ticks) {
+        load(q);
+        for (int t = 0; t < ticks; t++) tick();
+        return this->q();
+    }
+};
+
+// Whether rollouts of ts-tick steps keep a continuation (ts must leave its
+// offsets inside the 360-tick rollout, in order).
+RSX_HD bool chains_fit(int ts) { return ts >= 1 && ts <= 120; }
+
+// Full 360-tick rollout from q0: the slices after 120, 240 and 360 ticks
+// (uu, uu/s) and, when chains_fit(ts), the continuation states at ts,
+// 120 + ts, 240 + ts and 360.
+RSX_HD void rollout_full(Roller& R, const BallQ& q0, int ts, V3* pos, V3* vel, BallQ* chain) {
+    R.load(q0);
     int slice = 0;
     for (int t = 1; t <= kPredTicks; t++) {
-        (void)arena_tick<0>(P, T, W);
+        R.tick();
+        if (t == ts) chain[0] = R.q();
+        if (t == 120 + ts) chain[1] = R.q();
+        if (t == 240 + ts) chain[2] = R.q();
         if (t == 120 || t == 240 || t == 360) {
-            pos[slice] = P.ball.xf.o * kBT2UU;
-            vel[slice] = P.ball.vel * kBT2UU;
+            pos[slice] = R.P.ball.xf.o * kBT2UU;
+            vel[slice] = R.P.ball.vel * kBT2UU;
             slice++;
         }
     }
+    chain[3] = R.q();
+}
+
+// The rollout from the ball the last one passed at ts ticks (chain[0]),
+// from its continuation: the new slices are the states it passed at 120 + ts
+// and 240 + ts and its 360-tick state advanced ts ticks; every continuation
+// state advances ts ticks.  Equal to rollout_full from chain[0], bit for bit.
+RSX_HD void rollout_continue(Roller& R, int ts, V3* pos, V3* vel, BallQ* chain) {
+    pos[0] = chain[1].p * kBT2UU;
+    vel[0] = chain[1].v * kBT2UU;
+    pos[1] = chain[2].p * kBT2UU;
+    vel[1] = chain[2].v * kBT2UU;
+    for (int k = 0; k < 4; k++) chain[k] = R.advance(chain[k], ts);
+    pos[2] = chain[3].p * kBT2UU;
+    vel[2] = chain[3].v * kBT2UU;
+}
+
+// Ball-only rollout of `ball` (engine units): its position (uu) and
+// velocity (uu/s) after 120, 240 and 360 ticks.
+RSX_HD void predict_ball(const Ball& ball, const TickCtx& T, TickScratch<0>& W, Arena<0>& P, V3* pos, V3* vel) {
+    Roller R{T, W, P};
+    BallQ chain[4];
+    rollout_full(R, ball_q(ball), 0, pos, vel, chain);
 }
 
 }  // namespace vanta

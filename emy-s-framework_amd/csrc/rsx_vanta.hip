@@ -30,27 +30,83 @@ __device__ inline void ball_only_ctx(TickCtx& T, const VantaArgs& V, gjk::Simple
 }
 
 // The ball-prediction rollouts (vantaArenaUpdateKernel's first half,
-// RocketSimCudaObs.cuh:92-121): one lane per arena clears the Vanta words of
-// an arena that was reset, reads its ball from the arena words, runs the
-// 360-tick ball-only rollout and stores the three slices.  A kernel of its
-// own: the ball-only tick needs far fewer registers and far less private
-// memory than the rows, so more wavefronts are in flight to hide its
-// latency.
+// RocketSimCudaObs.cuh:92-121), in two passes so that no wavefront runs the
+// full 360 ticks for one lane while its others idle.  k_vanta_pred_step, one
+// lane per arena: clears the Vanta words of an arena that was reset; keeps the
+// slices when neither the tick nor the ball moved since the last rollout;
+// when the ball is, bit for bit, the one the last rollout passed ts ticks
+// after its start, continues that rollout (4 x ts ticks, rollout_continue:
+// the same slices as a new rollout); otherwise lists the arena for
+// k_vanta_pred_full, which runs the full rollouts of the listed arenas, one
+// per lane, in dense wavefronts.  Both read the ball from the arena words.
+__device__ inline vanta::BallQ arena_ball_q(const Col& col, int ball_base) {
+    return vanta::BallQ{col.getv(ball_base + 0), col.getv(ball_base + 3), col.getv(ball_base + 6)};
+}
+
 template <int MAXC>
-__global__ void __launch_bounds__(64) k_vanta_pred(BucketArgs B, uint32_t* vwords, const uint8_t* reset, VantaArgs V) {
+__device__ void store_rollout(const Col& vc, uint64_t tick, const vanta::BallQ& start, const V3* pos, const V3* vel,
+                              const vanta::BallQ* chain, bool chains) {
+    for (int k = 0; k < 3; k++) {
+        vc.setv(vanta::kVaPredPos + 3 * k, pos[k]);
+        vc.setv(vanta::kVaPredVel + 3 * k, vel[k]);
+    }
+    vc.set64(vanta::kVaT0, tick);
+    vanta::set_q(vc, vanta::kVaStart, start);
+    for (int k = 0; k < 4; k++) vanta::set_q(vc, vanta::kVaChain + 9 * k, chain[k]);
+    const uint32_t hi = vc.get(vanta::kVaAvailHi) & ~vanta::kVaChainsBit;
+    vc.set(vanta::kVaAvailHi, hi | (chains ? vanta::kVaChainsBit : 0u));
+}
+
+__global__ void k_vanta_count_zero(int* count) {
+    if (threadIdx.x == 0) *count = 0;
+}
+
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_vanta_pred_step(BucketArgs B, uint32_t* vwords, const uint8_t* reset,
+                                                        VantaArgs V, int* work, int* count) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= B.n) return;
     const Col col{B.words + a, B.n};
     const Col vc{vwords + a, B.n};
     if (reset && reset[B.global[a]]) vanta::clear<MAXC>(vc);
-    Ball ball;
-    ball.xf = Xf{m3_identity(), col.getv(Layout<MAXC>::kBallBase + 0)};
-    ball.vel = col.getv(Layout<MAXC>::kBallBase + 3);
-    ball.ang = col.getv(Layout<MAXC>::kBallBase + 6);
-    ball.force = ball.vel_cache = V3{0, 0, 0};
-    ball.interp = ball.xf;
-    ball.sleeping = false;
-    ball.bp_reg = -1;
+    const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
+    const uint64_t tick = col.get64(Layout<MAXC>::kArenaBase + kArTickLo);
+    const bool chains = (vc.get(vanta::kVaAvailHi) & vanta::kVaChainsBit) != 0;
+    const uint64_t t0 = vc.get64(vanta::kVaT0);
+    const int ts = V.Q.ticks_between;
+    if (chains && tick == t0 && vanta::same_ball(now, vanta::get_q(vc, vanta::kVaStart))) return;  // nothing moved
+    if (chains && vanta::chains_fit(ts) && tick == t0 + (uint64_t)ts &&
+        vanta::same_ball(now, vanta::get_q(vc, vanta::kVaChain))) {
+        vanta::BallQ chain[4];
+        for (int k = 0; k < 4; k++) chain[k] = vanta::get_q(vc, vanta::kVaChain + 9 * k);
+        gjk::Simplex sx;
+        TickCtx T;
+        ball_only_ctx(T, V, &sx);
+        TickScratch<0> W;
+        W.cull_age = 1 << 30;
+        W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
+        W.L.overflow = false;
+        Arena<0> P;
+        vanta::Roller R{T, W, P};
+        V3 pos[3], vel[3];
+        vanta::rollout_continue(R, ts, pos, vel, chain);
+        store_rollout<MAXC>(vc, tick, now, pos, vel, chain, true);
+        return;
+    }
+    work[atomicAdd(count, 1)] = a;
+}
+
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_vanta_pred_full(BucketArgs B, uint32_t* vwords, VantaArgs V, const int* work,
+                                                        const int* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const int a = work[i];
+    const Col col{B.words + a, B.n};
+    const Col vc{vwords + a, B.n};
+    const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
+    const uint64_t tick = col.get64(Layout<MAXC>::kArenaBase + kArTickLo);
+    const int ts = V.Q.ticks_between;
     gjk::Simplex sx;
     TickCtx T;
     ball_only_ctx(T, V, &sx);
@@ -59,12 +115,12 @@ __global__ void __launch_bounds__(64) k_vanta_pred(BucketArgs B, uint32_t* vword
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
     W.L.overflow = false;
     Arena<0> P;
+    vanta::Roller R{T, W, P};
     V3 pos[3], vel[3];
-    vanta::predict_ball(ball, T, W, P, pos, vel);
-    for (int k = 0; k < 3; k++) {
-        vc.setv(vanta::kVaPredPos + 3 * k, pos[k]);
-        vc.setv(vanta::kVaPredVel + 3 * k, vel[k]);
-    }
+    vanta::BallQ chain[4];
+    const bool fit = vanta::chains_fit(ts);
+    vanta::rollout_full(R, now, fit ? ts : 0, pos, vel, chain);
+    store_rollout<MAXC>(vc, tick, now, pos, vel, chain, fit);
 }
 
 // The rows: the arena update (pads, demo timers), then every car's coyote
@@ -96,14 +152,23 @@ __global__ void __launch_bounds__(256) k_vanta_clear(uint32_t* vwords, int n, in
 }
 
 hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const uint8_t* reset, const VantaArgs& v,
-                        hipStream_t st) {
+                        int* work, int* count, hipStream_t st) {
     if (!b.n) return hipSuccess;
     dim3 grid((b.n + 63) / 64), block(64);
     if (v.Q.predict) {
         // the rollouts clear the reset arenas' words first
-        if (maxc == 2) hipLaunchKernelGGL(k_vanta_pred<2>, grid, block, 0, st, b, vwords, reset, v);
-        else if (maxc == 4) hipLaunchKernelGGL(k_vanta_pred<4>, grid, block, 0, st, b, vwords, reset, v);
-        else hipLaunchKernelGGL(k_vanta_pred<6>, grid, block, 0, st, b, vwords, reset, v);
+        hipLaunchKernelGGL(k_vanta_count_zero, dim3(1), dim3(64), 0, st, count);
+#define RSX_LAUNCH_PRED(M)                                                                                       \
+    hipLaunchKernelGGL(k_vanta_pred_step<M>, grid, block, 0, st, b, vwords, reset, v, work, count);            \
+    hipLaunchKernelGGL(k_vanta_pred_full<M>, grid, block, 0, st, b, vwords, v, (const int*)work, (const int*)count)
+        if (maxc == 2) {
+            RSX_LAUNCH_PRED(2);
+        } else if (maxc == 4) {
+            RSX_LAUNCH_PRED(4);
+        } else {
+            RSX_LAUNCH_PRED(6);
+        }
+#undef RSX_LAUNCH_PRED
         const hipError_t r = hipGetLastError();
         if (r != hipSuccess) return r;
         reset = nullptr;

@@ -65,6 +65,8 @@ struct MeshView {
     const int* cell_start;     // uniform grid CSR: [num_cells + 1]
     const int* cell_tris;      // triangle ids
     const GridEntry* cell_ent; // the same lists as gather entries
+    const int* blk_start;      // per grid cell c: CSR of the 2 x 2 x 2 block of cells whose lowest corner is c
+    const GridEntry* blk_ent;  // the block's distinct triangles, sorted by packed key
     const uint8_t* susp_world; // suspension grid cells with world geometry (bled), 32*56*8
     const uint16_t* bp_meshes; // broadphase grid: per cell, mask of meshes listed in the cell
     const float* mesh_aabb;    // [16][6] shape AABB min xyz, max xyz (BT)

@@ -441,7 +441,7 @@ extern "C" void cc_vanta_build(void* p, void* v, const float* pred18, float dt, 
             }
         P.mirror[i] = (int8_t)m;
     }
-    vanta::Params Q{dt, deadzone, only_closest, pred18 != nullptr};
+    vanta::Params Q{dt, deadzone, only_closest, pred18 != nullptr, 8};
     vanta::State<6> S;
     vanta::load<6>(c, S);
     if (pred18) {
@@ -457,4 +457,51 @@ extern "C" void cc_vanta_build(void* p, void* v, const float* pred18, float dt, 
     const uint32_t lt = vanta::last_touch_id<6>(a->A);
     for (int i = 0; i < a->A.ncars; i++) vanta::build_row<6>(a->A, S, P, Q, i, lt, obs + (int64_t)i * vanta::kRow);
     vanta::store<6>(c, S, a->A.ncars);
+}
+
+// The rollout continuation against new rollouts: a full rollout from the
+// arena's ball keeps its continuation (ts ticks between builds); then, `steps`
+// times, the rollout from the ball it passed at ts ticks is made both by
+// continuation and anew.  Returns the number of steps whose slices and new
+// continuation states agree bit for bit.
+extern "C" int cc_vanta_continue_check(void* p, int ts, int steps) {
+    auto* a = (CcArena*)p;
+    TickCtx T;
+    T.cg = a->world->w.cg;
+    T.bg = a->world->w.bg;
+    T.mv = a->world->mv;
+    T.pads = a->world->w.pads;
+    T.dt = 1.f / kTickRate;
+    gjk::Simplex sx;
+    T.sx = &sx;
+    T.cull = sdf::Cull{};
+    T.lds_act = nullptr;
+    T.own = true;
+    T.pool = nullptr;
+    auto* W = new TickScratch<0>();
+    W->cull_age = 1 << 30;
+    W->L.overflow = false;
+    auto* P = new Arena<0>();
+    vanta::Roller R{T, *W, *P};
+    V3 pos[3], vel[3];
+    vanta::BallQ chain[4];
+    vanta::rollout_full(R, vanta::ball_q(a->A.ball), ts, pos, vel, chain);
+    int ok = 0;
+    for (int s = 0; s < steps; s++) {
+        const vanta::BallQ next = chain[0];
+        V3 pc[3], vc[3], pf[3], vf[3];
+        vanta::BallQ cc[4] = {chain[0], chain[1], chain[2], chain[3]}, cf[4];
+        vanta::rollout_continue(R, ts, pc, vc, cc);
+        vanta::rollout_full(R, next, ts, pf, vf, cf);
+        bool same = true;
+        for (int k = 0; k < 3; k++)
+            same = same && vanta::same_ball(vanta::BallQ{pc[k], vc[k], V3{0, 0, 0}}, vanta::BallQ{pf[k], vf[k], V3{0, 0, 0}});
+        for (int k = 0; k < 4; k++) same = same && vanta::same_ball(cc[k], cf[k]);
+        if (!same) break;
+        ok++;
+        for (int k = 0; k < 4; k++) chain[k] = cc[k];
+    }
+    delete P;
+    delete W;
+    return ok;
 }

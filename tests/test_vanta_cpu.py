@@ -464,3 +464,20 @@ def test_ball_only_tick_equals_the_arena_tick_without_cars(cpucheck):
             full.append([s.pos.x, s.pos.y, s.pos.z, s.vel.x, s.vel.y, s.vel.z])
         lib.cc_arena_free(h)
         np.testing.assert_array_equal(np.array(pred, dtype=F).reshape(3, 6), np.array(full, dtype=F))
+
+
+def test_rollout_continuation_equals_a_new_rollout(cpucheck):
+    """The kernel continues the last rollout instead of starting a new one
+    when the ball is, bit for bit, the one it passed after ts ticks: the
+    slices (the states it passed at 120 + ts and 240 + ts, and its 360-tick
+    state advanced ts ticks) and the advanced continuation states equal a new
+    rollout's, bit for bit, step after step (balls in flight, rolling and at
+    walls, ts = 8 and 1)."""
+    lib, w = cpucheck
+    rng = np.random.default_rng(13)
+    for k, b in enumerate(_prediction_states(rng, 24)):
+        h = C.c_void_p(lib.cc_arena_new(w, 0, 0))
+        lib.cc_set_ball(h, C.byref(b))
+        for ts in (8, 1):
+            assert lib.cc_vanta_continue_check(h, ts, 6) == 6, (k, ts)
+        lib.cc_arena_free(h)
