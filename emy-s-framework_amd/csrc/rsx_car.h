@@ -105,10 +105,9 @@ RSX_HD Xf car_child_xf(const CarGeom& g, const Xf& car) {
 // the last world step (Car::bp_reg).
 template <int MAXC>
 RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g, const BallGeom& bg, int self,
-                      V3 from, V3 to, RayHit& hit, gjk::Simplex& sx, bool with_mesh) {
-    hit.object = -1;
-    hit.fraction = 1.0f;
-    ray_static_world(mv, from, to, hit, with_mesh);
+                      V3 from, V3 to, RayHit& hit, gjk::Simplex& sx, const RayHit& mesh_hit) {
+    hit = mesh_hit;  // the mesh part, cast for the car's four rays together (ray_mesh4)
+    ray_planes(from, to, hit);
     const int from_cell = bp_cell_index(mv, from);  // btRSBroadphase::rayTest: the start cell's list
     const V3 seg = to - from;
     const float seg2 = dot(seg, seg);
@@ -152,10 +151,20 @@ RSX_HD void ray_world(const MeshView& mv, const Arena<MAXC>& A, const CarGeom& g
     if (hit.object >= 2 && !A.car[hit.object - 2].active) hit.object = -1;
 }
 
-// btVehicleRL::rayCast (btVehicleRL.cpp:118-212)
+// The segment of wheel i's suspension ray (btVehicleRL::rayCast's source and
+// target), with wheel_update_ws's hardpoint and direction.
+RSX_HD void wheel_ray_segment(const Car& c, const CarGeom& g, int i, V3& source, V3& target) {
+    const float travel = (kMaxSuspensionTravel * kUU2BT * 100) / 100;
+    const float real_len = g.rest_len[i] + travel + g.wheel_radius[i] - kSuspensionSubtraction;
+    source = xform(c.xf, g.wheel_cs[i]);
+    target = source + (c.xf.b * V3{0, 0, -1}) * real_len;
+}
+
+// btVehicleRL::rayCast (btVehicleRL.cpp:118-212); mesh_hit is the ray's hit
+// on the mesh triangles (object -1: none, or not cast).
 template <int MAXC>
 RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, int i,
-                          float solver_dt, gjk::Simplex& sx, bool with_mesh) {
+                          float solver_dt, gjk::Simplex& sx, const RayHit& mesh_hit) {
     Car& c = A.car[ci];
     wheel_update_ws(c, g, i);
     Wheel& w = c.wheel[i];
@@ -168,7 +177,7 @@ RSX_HD void wheel_raycast(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGe
     w.contact_point = target;
     w.ground = -1;
     RayHit hit;
-    ray_world(mv, A, g, bg, ci, source, target, hit, sx, with_mesh);
+    ray_world(mv, A, g, bg, ci, source, target, hit, sx, mesh_hit);
     if (hit.object >= 0) {
         float s = 1.0f - hit.fraction;
         w.contact_point = V3{s * source.x + hit.fraction * target.x, s * source.y + hit.fraction * target.y,
@@ -661,7 +670,19 @@ RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeo
     Car& c = A.car[ci];
     // updateVehicleFirst
     for (int i = 0; i < 4; i++) wheel_update_transform(c, g, i);
-    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx, rays_with_mesh);
+    // the mesh part of the four suspension rays, in one pass (a car the
+    // learned cull has shown to be out of the mesh's reach casts none)
+    RayHit mesh_hit[4];
+    {
+        V3 from[4], to[4];
+        for (int i = 0; i < 4; i++) {
+            wheel_ray_segment(c, g, i, from[i], to[i]);
+            mesh_hit[i].object = -1;
+            mesh_hit[i].fraction = 1.0f;
+        }
+        if (rays_with_mesh) ray_mesh4(mv, from, to, mesh_hit);
+    }
+    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx, mesh_hit[i]);
     vehicle_friction_impulses(A, c, g, dt, bg);
 
     bool jump_pressed = c.controls.jump && !c.last_controls.jump;

@@ -1253,7 +1253,7 @@ rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t) {
     const LaunchShape s{t->block_lanes, t->arenas_per_wave, t->pool_narrowphase};
     if (!shape_valid(s))
         return fail(RSX_INVALID_ARG,
-                    "block_lanes must be 64, 128 or 256, arenas_per_wave 64, 32, 16 or 8, pool_narrowphase 0 or 1");
+                    "block_lanes must be 64, 128, 256 or 512, arenas_per_wave 64, 32, 16 or 8, pool_narrowphase 0 or 1");
     HIP_TRY(hipSetDevice(e->device));
     RSX_TRY(prepare_shape(e, s));
     e->shape = s;
@@ -1275,13 +1275,13 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (!device_actions || env_steps < 1) return fail(RSX_INVALID_ARG, "need device actions and env_steps >= 1");
     HIP_TRY(hipSetDevice(e->device));
-    static const int kBlocks[] = {64, 128, 256};
+    static const int kBlocks[] = {64, 128, 256, 512};
     static const int kApw[] = {64, 32, 16, 8};
     std::vector<rsx_tune_result> res;
     for (int pool = 1; pool >= 0; pool--)
         for (int bl : kBlocks)
             for (int apw : kApw) res.push_back(rsx_tune_result{{bl, apw, pool}, -1.f, 0});
-    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 24");
+    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 32");
     // Snapshot of everything a trial overwrites, restored after each trial
     // and at the end.
     std::vector<std::pair<void*, size_t>> bufs;

@@ -20,6 +20,7 @@ hipError_t launch_env(int maxc, const BucketArgs& b, const WorldArgs& w, const E
     const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0, w.pool != 0);
     const size_t lds = per_wave * (s.block / 64);
     dim3 grid(blocks_for_waves(arena_waves(b.n, s), s)), block(s.block);
+    if (s.block == 512) return launch_env_wide(maxc, b, w, v, s, st);
     if (maxc == 2)
         hipLaunchKernelGGL(k_env_step<2>, grid, block, lds, st, b, w, v, s.arenas_per_wave, (int)per_wave);
     else if (maxc == 4)
@@ -33,6 +34,7 @@ hipError_t prepare_env_kernels(size_t bytes) {
     hipError_t e = set_lds_limit(k_env_step<2>, bytes, g_lds_done[0]);
     if (e == hipSuccess) e = set_lds_limit(k_env_step<4>, bytes, g_lds_done[1]);
     if (e == hipSuccess) e = set_lds_limit(k_env_step<6>, bytes, g_lds_done[2]);
+    if (e == hipSuccess) e = prepare_env_wide_kernels(bytes);
     return e;
 }
 

@@ -14,9 +14,13 @@ namespace {
 size_t g_lds_done[3] = {65536, 65536, 65536};
 }
 
-hipError_t launch_ticks(int maxc, const BucketArgs& b, const WorldArgs& w, int ticks, const LaunchShape& s,
+hipError_t launch_ticks(int maxc, const BucketArgs& b, const WorldArgs& w, int ticks, const LaunchShape& shape,
                         hipStream_t st) {
     if (!b.n) return hipSuccess;
+    // the tick path (parity, savestates) runs 512-lane shapes as 256-lane
+    // workgroups: the same wavefronts with the same arenas each
+    LaunchShape s = shape;
+    if (s.block > 256) s.block = 256;
     const size_t per_wave = arena_lds_per_wave(w.cull.enabled != 0, w.pool != 0);
     const size_t lds = per_wave * (s.block / 64);
     dim3 grid(blocks_for_waves(arena_waves(b.n, s), s)), block(s.block);
@@ -55,7 +59,7 @@ size_t arena_pool_offset() { return kPoolOffset; }
 int arena_waves(int n, const LaunchShape& s) { return (n + s.arenas_per_wave - 1) / s.arenas_per_wave; }
 
 bool shape_valid(const LaunchShape& s) {
-    const bool block_ok = s.block == 64 || s.block == 128 || s.block == 256;
+    const bool block_ok = s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512;
     const int a = s.arenas_per_wave;
     return block_ok && (a == 64 || a == 32 || a == 16 || a == 8) && (s.pool == 0 || s.pool == 1);
 }

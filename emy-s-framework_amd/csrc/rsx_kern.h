@@ -116,6 +116,14 @@ hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const u
 // Clears the Vanta words of every arena of a bucket (only < 0) or of arena `only`.
 hipError_t launch_vanta_clear(int maxc, uint32_t* vwords, int n, int only, hipStream_t st);
 
+// The same launches in 512-lane workgroups (rsx_k_env_wide.hip, rsx_k_mixed_wide.hip).
+hipError_t launch_env_wide(int maxc, const BucketArgs& b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
+                           hipStream_t st);
+hipError_t launch_env_mixed_wide(const BucketArgs* b, const WorldArgs& w, const EnvArgs& v, const LaunchShape& s,
+                                 hipStream_t st);
+hipError_t prepare_env_wide_kernels(size_t bytes);
+hipError_t prepare_mixed_wide_kernel(size_t bytes);
+
 // Sets each arena kernel's dynamic-LDS limit to `bytes` (needed above 64 KiB);
 // call outside stream capture.
 hipError_t prepare_arena_kernels(size_t bytes);

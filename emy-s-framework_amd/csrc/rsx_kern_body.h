@@ -168,6 +168,11 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     V.term[B.global[a]] = tt;
 }
 
+// One wavefront of the launch over every team-size bucket: the wavefront's
+// index picks its bucket, 3v3 wavefronts first, then 2v2, then 1v1.
+__device__ __forceinline__ void mixed_wave(const BucketArgs& b2, const BucketArgs& b4, const BucketArgs& b6, int w6,
+                                           int w4, const WorldArgs& G, const EnvArgs& V, int apw, int lds_wave);
+
 // The workgroup's first wavefront index and this wavefront's LDS.
 __device__ inline int wave_index() { return blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); }
 
@@ -185,6 +190,23 @@ hipError_t set_lds_limit(F kernel, size_t bytes, size_t& done) {
     hipError_t e = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e == hipSuccess) done = bytes;
     return e;
+}
+
+__device__ __forceinline__ void mixed_wave(const BucketArgs& b2, const BucketArgs& b4, const BucketArgs& b6, int w6,
+                                           int w4, const WorldArgs& G, const EnvArgs& V, int apw, int lds_wave) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint8_t* mine = lds + (threadIdx.x >> 6) * lds_wave;
+    int w = wave_index();
+    if (w < w6) {
+        env_wave<6>(b6, G, V, w, apw, mine);
+        return;
+    }
+    w -= w6;
+    if (w < w4) {
+        env_wave<4>(b4, G, V, w, apw, mine);
+        return;
+    }
+    env_wave<2>(b2, G, V, w - w4, apw, mine);
 }
 
 }  // namespace rsx

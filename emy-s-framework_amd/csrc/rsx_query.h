@@ -249,12 +249,87 @@ RSX_HD void ray_mesh(const MeshView& w, V3 from, V3 to, RayHit& hit) {
             }
 }
 
+// The mesh part of four rays at once (a car's suspension rays): their
+// segments' union spans at most two grid cells per axis, so the 2 x 2 x 2
+// block at its lowest cell lists every triangle any of them can meet, once.
+// One scan of that list tests each entry against every ray whose box it
+// meets, keeping per ray the smallest (fraction, packed key) -- the same set
+// of triangles and the same order as ray_mesh for each ray alone, so the
+// same hits; a car over an empty block costs one lookup instead of four grid
+// walks.  hit[k] holds what ray k hit before (or object -1).
+RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* hit) {
+    V3 mn[4], mx[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        mn[k] = vmin(from[k], to[k]);
+        mx[k] = vmax(from[k], to[k]);
+    }
+    const V3 umn = vmin(vmin(mn[0], mn[1]), vmin(mn[2], mn[3]));
+    const V3 umx = vmax(vmax(mx[0], mx[1]), vmax(mx[2], mx[3]));
+    const float inv = 1.0f / w.grid_cell;
+    const int x0 = clampi((int)floorf((umn.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y0 = clampi((int)floorf((umn.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z0 = clampi((int)floorf((umn.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    const int x1 = clampi((int)floorf((umx.x - w.grid_min.x) * inv), 0, w.gx - 1);
+    const int y1 = clampi((int)floorf((umx.y - w.grid_min.y) * inv), 0, w.gy - 1);
+    const int z1 = clampi((int)floorf((umx.z - w.grid_min.z) * inv), 0, w.gz - 1);
+    if (x1 - x0 > 1 || y1 - y0 > 1 || z1 - z0 > 1) {
+        for (int k = 0; k < 4; k++) ray_mesh(w, from[k], to[k], hit[k]);
+        return;
+    }
+    float frac[4];
+    int best[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        frac[k] = hit[k].object >= 0 ? hit[k].fraction : 1.0f;
+        best[k] = -1;
+    }
+    const int blk = (x0 * w.gy + y0) * w.gz + z0;
+    const int s = w.blk_start[blk], e = w.blk_start[blk + 1];
+    for (int j = s; j < e; j++) {
+        const GridEntry g = w.blk_ent[j];
+        bool ov[4];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            ov[k] = !(g.mn[2] > mx[k].z || g.mx[2] < mn[k].z) && !(g.mn[0] > mx[k].x || g.mx[0] < mn[k].x) &&
+                    !(g.mn[1] > mx[k].y || g.mx[1] < mn[k].y);
+            any = any || ov[k];
+        }
+        if (!any) continue;
+        V3 a, b, c;
+        load_tri(w, cand_tri(g.key), a, b, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!ov[k]) continue;
+            V3 nrm;
+            float d;
+            if (!ray_triangle_hit(from[k], to[k], a, b, c, d, nrm)) continue;
+            if (d < frac[k] || (d == frac[k] && best[k] >= 0 && g.key < best[k])) {
+                frac[k] = d;
+                best[k] = g.key;
+                hit[k].fraction = d;
+                hit[k].normal = nrm;
+                hit[k].object = 0;
+            }
+        }
+    }
+}
+
+// The four static planes' part of the Bullet ray test (after the mesh);
+// a plane hit replaces only a strictly closer one.
+RSX_HD void ray_planes(V3 from, V3 to, RayHit& hit);
+
 // Full Bullet ray test against the static world (mesh triangles near the ray,
 // then the four planes), closest hit wins (strictly closer replaces).
 // with_mesh = false leaves out the mesh triangles, for a ray the learned cull
 // has shown to be out of their reach.
 RSX_HD void ray_static_world(const MeshView& w, V3 from, V3 to, RayHit& hit, bool with_mesh = true) {
     if (with_mesh) ray_mesh(w, from, to, hit);
+    ray_planes(from, to, hit);
+}
+
+RSX_HD void ray_planes(V3 from, V3 to, RayHit& hit) {
     float frac = hit.object >= 0 ? hit.fraction : 1.0f;
     for (int p = 0; p < kNumPlanes; p++) {
         // btStaticPlaneShape::processAllTriangles around the ray's local AABB,

@@ -290,7 +290,10 @@ rsx_status rsx_synchronize(rsx_engine* e);
 /* Launch shape of the arena kernels (rsx_step_env, rsx_step_ticks).  The
  * reference fixes it at 128 threads per block, one thread per arena
  * (RocketSimCudaApi.cuh:553-554).  Here:
- *   block_lanes       lanes per workgroup: 64, 128 or 256;
+ *   block_lanes       lanes per workgroup: 64, 128, 256 or 512 (512: two
+ *                     wavefronts per SIMD with at most 256 registers a lane;
+ *                     its LDS fits only without pooling; the tick path runs
+ *                     it as 256-lane workgroups);
  *   arenas_per_wave   arenas per 64-lane wavefront: 64, 32, 16 or 8 (below 64
  *                     the spare lanes repeat a lane's arena and store nothing,
  *                     which trades lanes for fewer diverging branches per
@@ -315,10 +318,11 @@ rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t);
 rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t);
 
 /* Autotuner: times `env_steps` env steps (after one untimed) of every shape
- * in {64, 128, 256} x {64, 32, 16, 8} x {pooled, unpooled} on the engine's
- * current arenas with the given device actions, keeps the fastest, and
+ * in {64, 128, 256, 512} x {64, 32, 16, 8} x {pooled, unpooled} on the
+ * engine's current arenas with the given device actions (shapes whose LDS
+ * does not fit a workgroup are reported infeasible), keeps the fastest, and
  * restores the arenas, the env outputs and the cull's counters to what they
- * were.  results (nullable, capacity >= 24) receives every shape's time;
+ * were.  results (nullable, capacity >= 32) receives every shape's time;
  * *count the number written.
  * Blocks the host until done. */
 rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t env_steps, rsx_tune_result* results,

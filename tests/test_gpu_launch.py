@@ -58,7 +58,8 @@ def test_launch_shapes_are_bit_identical(torch_cuda):
     acts = _actions(torch, b.num_cars, 11, 3)
     want = _run(b, acts)
     b.Destroy()
-    for shape in ((64, 64, 0), (128, 64, 1), (128, 32, 0), (256, 16, 1), (256, 8, 1), (64, 8, 0)):
+    for shape in ((64, 64, 0), (128, 64, 1), (128, 32, 0), (256, 16, 1), (256, 8, 1), (64, 8, 0), (512, 64, 0),
+                  (512, 16, 0)):
         b = _stress_batch(n)
         got = _run(b, acts, shape)
         assert b.GetTuning() == shape
@@ -69,7 +70,7 @@ def test_launch_shapes_are_bit_identical(torch_cuda):
 
 def test_invalid_shapes_are_rejected(torch_cuda):
     b = E.RocketSimBatch(4, 1)
-    for shape in ((512, 64, 1), (96, 64, 1), (64, 48, 1), (64, 0, 1), (64, 64, 2)):
+    for shape in ((1024, 64, 0), (96, 64, 1), (64, 48, 1), (64, 0, 1), (64, 64, 2), (512, 64, 1)):
         with pytest.raises(E.RsxError):
             b.SetTuning(*shape)
     assert b.GetTuning() == (64, 64, 1)
@@ -236,7 +237,8 @@ def test_autotune_restores_state_and_keeps_the_fastest(torch_cuda):
     before = _snapshot(b)
     masks_before = bytes(b.CopyBuiltDefaultActionMasks())
     res = b.Autotune(d.data_ptr(), env_steps=1)
-    assert len(res) == 24
+    assert len(res) == 32
+    assert any(r[0] == 512 and r[3] is not None for r in res)  # unpooled 512-lane workgroups fit
     feas = [r for r in res if r[3] is not None]
     assert feas and all(r[3] > 0 for r in feas)
     best = min(feas, key=lambda r: r[3])
