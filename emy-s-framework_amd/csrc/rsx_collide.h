@@ -47,6 +47,7 @@ struct TickCtx {
     uint16_t* lds_act; // the wavefront's LDS activation buffer for the cull's network
     bool own;          // this lane stores its arena (false: it mirrors another lane, or pads the grid)
     uint8_t* pool;     // the wavefront's LDS for pooled box-triangle queries (null: each lane runs its own)
+    float* slv = nullptr;  // this lane's column of the wavefront's LDS for the solver's body state (null: private)
 };
 
 // --- manifold bookkeeping ----------------------------------------------------

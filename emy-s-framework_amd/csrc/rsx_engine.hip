@@ -265,7 +265,8 @@ BucketArgs bucket_args(const Bucket& b) {
 }
 
 WorldArgs world_args(const rsx_engine* e, const LaunchShape& s) {
-    return WorldArgs{e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->cull, s.pool};
+    return WorldArgs{e->hw.cg, e->hw.bg, e->dview, e->dw.pads, e->cull, s.pool,
+                     (int)arena_lds_per_wave(e->cull.enabled != 0, s.pool != 0)};
 }
 
 // Checks that the shape's LDS fits a workgroup and raises the kernels'

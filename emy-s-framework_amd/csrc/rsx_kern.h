@@ -54,6 +54,7 @@ struct WorldArgs {
     const PadGeom* pads;
     sdf::Cull cull;
     int pool;                  // pool each wavefront's box-triangle queries (car_gjk_pool)
+    int lds_bytes;             // each wavefront's LDS share (arena_lds_per_wave)
 };
 
 // The env half of a fused env step.

@@ -39,6 +39,14 @@ __device__ inline void tick_ctx(TickCtx& T, const WorldArgs& G, uint8_t* lds_wav
     T.cull = G.cull;
     T.own = own;
     T.pool = G.pool ? lds_wave + pool_off : nullptr;
+    T.slv = nullptr;
+}
+
+// The solver keeps its bodies' hot state in the wavefront's LDS share when
+// the share holds it (the simplices and pooled queries are idle meanwhile).
+template <int MAXC>
+__device__ inline void solver_lds(TickCtx& T, const WorldArgs& G, uint8_t* lds_wave) {
+    T.slv = G.lds_bytes >= solver_lds_bytes<MAXC>() ? (float*)lds_wave + (threadIdx.x & 63) : nullptr;
 }
 
 // `ticks` ticks of the bucket's arenas under their stored controls
@@ -60,6 +68,7 @@ __device__ __forceinline__ void ticks_wave(const BucketArgs& B, const WorldArgs&
     }
     TickCtx T;
     tick_ctx(T, G, lds_wave, la.own, kPoolOffset);
+    solver_lds<MAXC>(T, G, lds_wave);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
@@ -113,6 +122,7 @@ __device__ __forceinline__ void env_wave(const BucketArgs& B, const WorldArgs& G
     if (V.rt_n > 0) snapshot_prev<MAXC>(A, P);
     TickCtx T;
     tick_ctx(T, G, lds_wave, la.own, kPoolOffset);
+    solver_lds<MAXC>(T, G, lds_wave);
     TickScratch<MAXC> W;
     W.cull_age = 1 << 30;  // the first tick of a launch runs the cull's network
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;

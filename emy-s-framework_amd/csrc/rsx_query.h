@@ -89,12 +89,20 @@ RSX_HD int gather_tris(const MeshView& w, V3 mn, V3 mx, int* out, int cap) {
         // arithmetic), the block lists it once, in key order
         const int blk = (x0 * w.gy + y0) * w.gz + z0;
         const int s = w.blk_start[blk], e = w.blk_start[blk + 1];
-        for (int k = s; k < e; k++) {
-            const GridEntry g = w.blk_ent[k];
-            if (g.mn[2] > mx.z || g.mx[2] < mn.z) continue;
-            if (g.mn[0] > mx.x || g.mx[0] < mn.x) continue;
-            if (g.mn[1] > mx.y || g.mx[1] < mn.y) continue;
-            if (n < cap) out[n++] = g.key;
+        // four entries per round, their loads issued together
+        for (int k = s; k < e; k += 4) {
+            GridEntry g[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (k + q < e) g[q] = w.blk_ent[k + q];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (k + q >= e) break;
+                if (g[q].mn[2] > mx.z || g[q].mx[2] < mn.z) continue;
+                if (g[q].mn[0] > mx.x || g[q].mx[0] < mn.x) continue;
+                if (g[q].mn[1] > mx.y || g[q].mx[1] < mn.y) continue;
+                if (n < cap) out[n++] = g[q].key;
+            }
         }
         return n;
     }
@@ -286,8 +294,16 @@ RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* h
     }
     const int blk = (x0 * w.gy + y0) * w.gz + z0;
     const int s = w.blk_start[blk], e = w.blk_start[blk + 1];
-    for (int j = s; j < e; j++) {
-        const GridEntry g = w.blk_ent[j];
+    for (int j0 = s; j0 < e; j0 += 2) {
+        // two entries per round, their loads issued together
+        GridEntry gg[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+            if (j0 + q < e) gg[q] = w.blk_ent[j0 + q];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+        if (j0 + q >= e) break;
+        const GridEntry& g = gg[q];
         bool ov[4];
         bool any = false;
 #pragma unroll
@@ -312,6 +328,7 @@ RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* h
                 hit[k].normal = nrm;
                 hit[k].object = 0;
             }
+        }
         }
     }
 }

@@ -290,6 +290,128 @@ RSX_HD void solver_iterate(Solver<MAXC>& S) {
     }
 }
 
+// ---- iterations with the bodies' hot state in LDS ---------------------------
+// The iterations read and update, per row, the two bodies' delta velocities
+// (or split-impulse push and turn), inverse mass and dynamic flag.  Kept in
+// private memory, every row waits for the previous row's stores to come back
+// from the cache; in LDS a lane's accesses complete in order at LDS latency.
+// Each lane owns a column of the wavefront's LDS: field f of body b at
+// col[(b * kHotFields + f) * 64].  The arithmetic is the same, in the same
+// order, as solve_split / solve_row on the private bodies.
+constexpr int kHotFields = 14;  // dlin 3, dang 3, push 3, turn 3, inv_mass, dyn
+template <int MAXC>
+constexpr int solver_lds_bytes() {
+    return (MAXC + 2) * kHotFields * 64 * (int)sizeof(float);
+}
+
+struct HotBodies {
+    float* col;
+    RSX_HD float& f(int b, int k) const { return col[(b * kHotFields + k) * 64]; }
+    RSX_HD V3 v(int b, int k) const { return V3{f(b, k), f(b, k + 1), f(b, k + 2)}; }
+    RSX_HD void set(int b, int k, V3 x) const {
+        f(b, k) = x.x;
+        f(b, k + 1) = x.y;
+        f(b, k + 2) = x.z;
+    }
+};
+constexpr int kHotDlin = 0, kHotDang = 3, kHotPush = 6, kHotTurn = 9, kHotInvMass = 12, kHotDyn = 13;
+
+RSX_HD void hot_apply(const HotBodies& H, int b, int lin_k, int ang_k, V3 lin_comp, V3 ang_comp, float imp) {
+    if (H.f(b, kHotDyn) != 0.f) {
+        H.set(b, lin_k, H.v(b, lin_k) + lin_comp * imp);
+        H.set(b, ang_k, H.v(b, ang_k) + ang_comp * imp);
+    }
+}
+
+RSX_HD float hot_solve_split(const HotBodies& H, Row& c) {
+    float delta = 0.f;
+    if (c.rhs_pen != 0.f) {
+        delta = c.rhs_pen - c.push * 0.f;
+        const float dv1 = dot(c.n1, H.v(c.a, kHotPush)) + dot(c.rc1, H.v(c.a, kHotTurn));
+        const float dv2 = dot(c.n2, H.v(c.b, kHotPush)) + dot(c.rc2, H.v(c.b, kHotTurn));
+        delta -= dv1 * c.jac_inv;
+        delta -= dv2 * c.jac_inv;
+        const float sum = c.push + delta;
+        if (sum < c.lo) {
+            delta = c.lo - c.push;
+            c.push = c.lo;
+        } else {
+            c.push = sum;
+        }
+        hot_apply(H, c.a, kHotPush, kHotTurn, c.n1 * H.f(c.a, kHotInvMass), c.angA, delta);
+        hot_apply(H, c.b, kHotPush, kHotTurn, c.n2 * H.f(c.b, kHotInvMass), c.angB, delta);
+    }
+    return (float)((double)delta * (1. / (double)c.jac_inv));
+}
+
+RSX_HD void hot_solve_row(const HotBodies& H, Row& c, bool upper) {
+    float delta = c.rhs - c.applied * 0.f;
+    const float dv1 = dot(c.n1, H.v(c.a, kHotDlin)) + dot(c.rc1, H.v(c.a, kHotDang));
+    const float dv2 = dot(c.n2, H.v(c.b, kHotDlin)) + dot(c.rc2, H.v(c.b, kHotDang));
+    delta -= dv1 * c.jac_inv;
+    delta -= dv2 * c.jac_inv;
+    const float sum = c.applied + delta;
+    if (sum < c.lo) {
+        delta = c.lo - c.applied;
+        c.applied = c.lo;
+    } else if (upper && sum > c.hi) {
+        delta = c.hi - c.applied;
+        c.applied = c.hi;
+    } else {
+        c.applied = sum;
+    }
+    hot_apply(H, c.a, kHotDlin, kHotDang, c.n1 * H.f(c.a, kHotInvMass), c.angA, delta);
+    hot_apply(H, c.b, kHotDlin, kHotDang, c.n2 * H.f(c.b, kHotInvMass), c.angB, delta);
+}
+
+// solver_iterate with the used bodies' hot state copied into the lane's LDS
+// column and back.
+template <int MAXC>
+RSX_HD void solver_iterate_lds(Solver<MAXC>& S, float* col) {
+    const HotBodies H{col};
+    for (int b = 0; b < MAXC + 2; b++) {
+        const SBody& s = S.body[b];
+        if (!s.used) continue;
+        H.set(b, kHotDlin, s.dlin);
+        H.set(b, kHotDang, s.dang);
+        H.set(b, kHotPush, s.push);
+        H.set(b, kHotTurn, s.turn);
+        H.f(b, kHotInvMass) = s.inv_mass;
+        H.f(b, kHotDyn) = s.dyn ? 1.f : 0.f;
+    }
+    for (int it = 0; it < kSolverIterations; it++) {
+        float res = 0.f;
+        for (int j = 0; j < S.nrow; j++) {
+            float r = hot_solve_split(H, S.row[j]);
+            res = maxf(res, r * r);
+        }
+        if (res <= 0.f || it >= (kSolverIterations - 1)) break;
+    }
+    for (int it = 0; it < kSolverIterations; it++) {
+        for (int j = 0; j < S.nrow; j++) {
+            if (S.row[j].special) continue;
+            hot_solve_row(H, S.row[j], false);
+        }
+        for (int j = 0; j < S.nfric; j++) {
+            Row& f = S.fric[j];
+            float total = S.row[f.fidx].applied;
+            if (total > 0.f) {
+                f.lo = -(f.friction * total);
+                f.hi = f.friction * total;
+                hot_solve_row(H, f, true);
+            }
+        }
+    }
+    for (int b = 0; b < MAXC + 2; b++) {
+        SBody& s = S.body[b];
+        if (!s.used) continue;
+        s.dlin = H.v(b, kHotDlin);
+        s.dang = H.v(b, kHotDang);
+        s.push = H.v(b, kHotPush);
+        s.turn = H.v(b, kHotTurn);
+    }
+}
+
 // btSolverBody::writebackVelocityAndTransform + writeBackBodies
 RSX_HD void sbody_writeback(SBody& s, bool no_rot, float dt, Xf& xf_out, V3& lin_out, V3& ang_out) {
     s.lin += s.dlin;

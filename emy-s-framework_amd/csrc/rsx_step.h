@@ -192,11 +192,16 @@ RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx
 }
 
 // Narrowphase of the ball against mesh `mesh` (convex-concave, swapped).
+// The candidates are sorted by mesh and the meshes visited in increasing
+// order, so the scan starts at `pos` (where the previous mesh's run ended)
+// and leaves it at the end of this mesh's run.
 template <int MAXC>
-RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh, const int* cand, int ncand) {
+RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh, const int* cand, int ncand,
+                         int& pos) {
     const Ball& b = A.ball;
     Xf trB = xf_identity();
-    for (int i = 0; i < ncand; i++) {
+    int i = pos;
+    for (; i < ncand; i++) {
         const int pk = cand[i];
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
@@ -211,6 +216,7 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh
             if (idx >= 0) contact_added(A, T, m, idx, t);
         }
     }
+    pos = i;
     manifold_refresh(m, b.xf, trB);
 }
 
@@ -327,7 +333,7 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
 // here, in the same order.
 template <int MAXC>
 RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
-                        int ncand) {
+                        int ncand, int& pos) {
   if constexpr (MAXC > 0) {
     gjk::GjkEpa& G = W.gj;
     const int* cand = W.cand;
@@ -339,7 +345,8 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
     const float max_d = g.box_margin + 0.f + m.breaking;
     const PooledContact* pooled = T.pool ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
-    for (int i = 0; i < ncand; i++) {
+    int i = pos;
+    for (; i < ncand; i++) {
         const int pk = cand[i];
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
@@ -369,6 +376,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
             if (idx >= 0) contact_added(A, T, m, idx, t);
         }
     }
+    pos = i;
     if (m.n) manifold_refresh(m, c.xf, trB);
   }
 }
@@ -496,6 +504,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         if (T.pool && d >= 1) car_gjk_pool(A, T, W, d - 1, mask, ncand);
 #endif
         if (demoed) continue;  // no contact response: never solved
+        int cpos = 0;  // the candidates not yet consumed by an earlier mesh
         for (int ms = 0; ms < kNumMeshes; ms++) {
             if (!((open >> ms) & 1)) continue;
             const float* ma = T.mv.mesh_aabb + ms * 6;
@@ -517,8 +526,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
                 if ((mask >> ms) & 1) {
-                    if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand);
-                    else car_vs_mesh(A, T, W, m, d - 1, ms, ncand);
+                    if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand, cpos);
+                    else car_vs_mesh(A, T, W, m, d - 1, ms, ncand, cpos);
                 }
             }
             mf_close(L, k);
@@ -739,7 +748,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         add_contact_friction(S, bi, FIX, normal, S.sp_friction[bi], rel1, rel2, ridx);
     }
 
-    solver_iterate(S);
+    if (T.slv) solver_iterate_lds(S, T.slv);
+    else solver_iterate(S);
 
     // write-back
     if (in_solver[0]) {

@@ -23,6 +23,9 @@ struct CcArena {
     Arena<6> A;
     TickScratch<6> W;
     int last_scored;
+    // when set, the solver runs its iterations on this buffer standing in
+    // for the lane's LDS column (solver_iterate_lds), as the GPU kernels do
+    std::vector<float> slv;
 };
 
 extern "C" {
@@ -145,6 +148,7 @@ int cc_step(void* p, int ticks) {
     T.lds_act = nullptr;
     T.own = true;
     T.pool = nullptr;
+    T.slv = a->slv.empty() ? nullptr : a->slv.data();
     int scored = 0;
     for (int t = 0; t < ticks; t++) scored = arena_tick(a->A, T, a->W) ? 1 : 0;
     a->last_scored = scored;
@@ -504,4 +508,11 @@ extern "C" int cc_vanta_continue_check(void* p, int ts, int steps) {
     delete P;
     delete W;
     return ok;
+}
+
+// Solver iterations on an LDS-like column (1) or on the private bodies (0).
+extern "C" void cc_set_solver_lds(void* p, int on) {
+    auto* a = (CcArena*)p;
+    if (on) a->slv.assign(solver_lds_bytes<6>() / sizeof(float), 0.f);
+    else a->slv.clear();
 }

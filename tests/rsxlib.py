@@ -156,6 +156,7 @@ def load_cpucheck():
         ("cc_vanta_clear", None, [VP]),
         ("cc_vanta_predict", None, [VP, C.POINTER(C.c_float)]),
         ("cc_vanta_continue_check", C.c_int, [VP, C.c_int, C.c_int]),
+        ("cc_set_solver_lds", None, [VP, C.c_int]),
         ("cc_vanta_build", None, [VP, VP, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
                                   C.POINTER(C.c_float)]),
     ])

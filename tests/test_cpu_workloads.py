@@ -173,3 +173,46 @@ def test_extra_cases_match_oracle(oracle, cpucheck):
     assert [r[0] for r in respawned] == ["demolition respawn"]
     auto = engs[3]
     assert any(c[0].is_auto_flipping for c in auto.cars)
+
+
+def _run_solver_variant(lib, world, case, ticks, lut, lds):
+    """The CPU build on one workload case, the solver iterating on the
+    private bodies (lds=0) or on an LDS-like column (lds=1)."""
+    import ctypes as C
+    import parity_harness as P
+    h = C.c_void_p(lib.cc_arena_new(world, case["blue"], case["orange"]))
+    lib.cc_set_solver_lds(h, lds)
+    lib.cc_set_tick(h, case["tick"])
+    for i, s in enumerate(case["cars"]):
+        lib.cc_set_car(h, i, C.byref(s))
+        lib.cc_set_wheels(h, i, P.WHEEL_ZERO6)
+    lib.cc_set_ball(h, C.byref(case["ball"]))
+    lib.cc_set_pads(h, case["pads"])
+    out = []
+    for t in range(ticks):
+        for i, c in enumerate(WH.case_controls(case, t, lut)):
+            lib.cc_set_controls(h, i, C.byref(c))
+        lib.cc_step(h, 1)
+        for i in range(case["blue"] + case["orange"]):
+            s = R.CarState()
+            lib.cc_get_car(h, i, C.byref(s))
+            out.append(bytes(s))
+        b = R.BallState()
+        lib.cc_get_ball(h, C.byref(b))
+        out.append(bytes(b))
+    lib.cc_arena_free(h)
+    return out
+
+
+def test_solver_on_an_lds_column_is_bit_identical(cpucheck):
+    """The GPU kernels run the solver's iterations on the bodies' hot state in
+    the wavefront's LDS (solver_iterate_lds); the same code on the CPU build,
+    over contact-heavy and mixed workload samples, gives every car and ball
+    bit for bit what the iterations on the private bodies give."""
+    lib, world = cpucheck
+    lut = WH.lut_controls()
+    for kind, count in (("stress", 96), ("mixed", 48)):
+        for k, case in enumerate(WH.sample(kind, count)):
+            a = _run_solver_variant(lib, world, case, 24, lut, 0)
+            b = _run_solver_variant(lib, world, case, 24, lut, 1)
+            assert a == b, (kind, k)
