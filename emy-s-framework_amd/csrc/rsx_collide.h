@@ -280,13 +280,23 @@ RSX_HD bool clamp_normal(V3 edge, V3 tri_normal, V3 n, float corrected, V3& out)
     return false;
 }
 
-RSX_HD void adjust_internal_edge(const MeshView& mv, int tri, Contact& cp) {
+// A corrected contact normal: the point on the mesh moves with it
+// (btAdjustInternalEdgeContacts; the mesh transform is the identity).
+RSX_HD void contact_renormal(Contact& cp, V3 n) {
+    cp.normal = n;
+    cp.posB = cp.posA - cp.normal * cp.dist;
+    cp.localB = inv_xform(xf_identity(), cp.posB);
+}
+
+// Returns whether the contact's normal was corrected (contact_renormal ran).
+RSX_HD bool adjust_internal_edge(const MeshView& mv, int tri, Contact& cp) {
     const float* info = mv.tri_info + (size_t)tri * 4;
     uint32_t flags = __float_as_uint_rsx(info[3]);
     const float angles[3] = {info[0], info[1], info[2]};
     const float kMaxEdgeAngle = 2.0f * kPi, kEdgeDist = 0.1f, kConvexEps = 0.f;
     if (!(fabsf(angles[0]) < kMaxEdgeAngle) && !(fabsf(angles[1]) < kMaxEdgeAngle) && !(fabsf(angles[2]) < kMaxEdgeAngle))
-        return;
+        return false;
+    bool renormal = false;
     V3 v[3];
     load_tri(mv, tri, v[0], v[1], v[2]);
     V3 tri_normal = normalized(cross(v[1] - v[0], v[2] - v[0]));
@@ -337,9 +347,8 @@ RSX_HD void adjust_internal_edge(const MeshView& mv, int tri, Contact& cp) {
             V3 clamped;
             if (clamp_normal(edge, sf * tri_normal, ln, angles[e], clamped)) {
                 if (dot(clamped, 1.f * tri_normal) > 0) {
-                    cp.normal = m3_identity() * clamped;
-                    cp.posB = cp.posA - cp.normal * cp.dist;
-                    cp.localB = inv_xform(xf_identity(), cp.posB);
+                    contact_renormal(cp, m3_identity() * clamped);
+                    renormal = true;
                 }
             }
         }
@@ -347,11 +356,11 @@ RSX_HD void adjust_internal_edge(const MeshView& mv, int tri, Contact& cp) {
     if (near_edge && concave_hits > 0) {
         V3 nn = tri_normal * 1.f;
         float d = dot(nn, local_n);
-        if (d < 0) return;
-        cp.normal = m3_identity() * nn;
-        cp.posB = cp.posA - cp.normal * cp.dist;
-        cp.localB = inv_xform(xf_identity(), cp.posB);
+        if (d < 0) return renormal;
+        contact_renormal(cp, m3_identity() * nn);
+        renormal = true;
     }
+    return renormal;
 }
 
 // --- sphere vs plane / triangle ------------------------------------------------

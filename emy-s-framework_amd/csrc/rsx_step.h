@@ -235,11 +235,13 @@ RSX_HD bool box_tri_separated(const Xf& child, const M3& inv_basis, V3 hm, const
     return false;
 }
 
-// Result of one pooled box-triangle query (gjk::closest_points).
+// Result of one pooled box-triangle query (gjk::closest_points), with the
+// internal-edge correction of the contact it adds (adjust_internal_edge).
 struct PooledContact {
     V3 normal_on_b, point_on_b;
     float depth;
-    int hit;
+    int hit;          // bit 0: a point within reach; bit 1: its normal is corrected to `normal_fix`
+    V3 normal_fix;
 };
 
 // LDS of a wavefront's pooled box-triangle queries: the 64 lanes' car child
@@ -317,6 +319,20 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
             r.normal_on_b = ct.normal_on_b;
             r.point_on_b = ct.point_on_b;
             r.depth = ct.depth;
+            if (r.hit && !(ct.depth > g.breaking)) {
+                // the point as manifold_add stores it, then contact_added's
+                // internal-edge correction, here instead of in the owner lane
+                Contact cp;
+                cp.posB = ct.point_on_b;
+                cp.posA = ct.point_on_b + ct.normal_on_b * ct.depth;
+                cp.localB = inv_xform(xf_identity(), ct.point_on_b);
+                cp.normal = ct.normal_on_b;
+                cp.dist = ct.depth;
+                if (adjust_internal_edge(T.mv, (int)(e & 0xffffu), cp)) {
+                    r.hit |= 2;
+                    r.normal_fix = cp.normal;
+                }
+            }
             res[it] = r;
         }
     }
@@ -351,23 +367,26 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
-        gjk::Shapes sh;
-        load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
         if (pooled) {
             // the wavefront ran this lane's queries already (car_gjk_pool)
             if (W.pidx[i] < 0) continue;
             const int it = W.pool_base + W.pidx[i];
             if (it < kPoolCap) {
-                const PooledContact r = pooled[it];
-                if (r.hit) {
+                const PooledContact& r = pooled[it];
+                const int hit = r.hit;
+                if (hit) {
                     int idx = manifold_add(m, c.xf, trB, r.normal_on_b, r.point_on_b, r.depth, 0.6f, 0.3f);
-                    if (idx >= 0) contact_added(A, T, m, idx, t);
+                    if (idx >= 0) {
+                        contact_added(A, T, m, idx);  // the edge correction ran with the query
+                        if (hit & 2) contact_renormal(m.pt[idx], r.normal_fix);
+                    }
                 }
                 continue;
             }
-        } else if (box_tri_separated(child, inv_basis, hm, sh.tri, m.breaking)) {
-            continue;
         }
+        gjk::Shapes sh;
+        load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
+        if (!pooled && box_tri_separated(child, inv_basis, hm, sh.tri, m.breaking)) continue;
         sh.h = g.box_implicit;
         sh.margin = g.box_margin;
         gjk::Contact1 ct;
