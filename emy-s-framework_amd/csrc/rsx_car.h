@@ -664,16 +664,18 @@ RSX_HD void car_update_auto_roll(Car& c, const CarGeom& g, float dt, int nwc) {
 }
 
 // Car::_PreTickUpdate for a car that is not demoed (demo handling is done by the caller).
+// `cast`: the mesh part of the four suspension rays, cast by the caller
+// (null: cast here, unless rays_with_mesh is false).
 template <int MAXC>
 RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeom& bg, const MeshView& mv, float dt,
-                         float solver_dt, gjk::Simplex& sx, bool rays_with_mesh = true) {
+                         float solver_dt, gjk::Simplex& sx, bool rays_with_mesh = true, const RayHit* cast = nullptr) {
     Car& c = A.car[ci];
     // updateVehicleFirst
     for (int i = 0; i < 4; i++) wheel_update_transform(c, g, i);
     // the mesh part of the four suspension rays, in one pass (a car the
     // learned cull has shown to be out of the mesh's reach casts none)
     RayHit mesh_hit[4];
-    {
+    if (!cast) {
         V3 from[4], to[4];
         for (int i = 0; i < 4; i++) {
             wheel_ray_segment(c, g, i, from[i], to[i]);
@@ -681,8 +683,9 @@ RSX_HD void car_pre_tick(Arena<MAXC>& A, int ci, const CarGeom& g, const BallGeo
             mesh_hit[i].fraction = 1.0f;
         }
         if (rays_with_mesh) ray_mesh4(mv, from, to, mesh_hit);
+        cast = mesh_hit;
     }
-    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx, mesh_hit[i]);
+    for (int i = 0; i < 4; i++) wheel_raycast(A, ci, g, bg, mv, i, solver_dt, sx, cast[i]);
     vehicle_friction_impulses(A, c, g, dt, bg);
 
     bool jump_pressed = c.controls.jump && !c.last_controls.jump;

@@ -265,13 +265,10 @@ RSX_HD void ray_mesh(const MeshView& w, V3 from, V3 to, RayHit& hit) {
 // of triangles and the same order as ray_mesh for each ray alone, so the
 // same hits; a car over an empty block costs one lookup instead of four grid
 // walks.  hit[k] holds what ray k hit before (or object -1).
-RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* hit) {
-    V3 mn[4], mx[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        mn[k] = vmin(from[k], to[k]);
-        mx[k] = vmax(from[k], to[k]);
-    }
+// The 2 x 2 x 2 block (the grid cell at its lowest corner) listing every
+// triangle four ray segments (boxes mn[k], mx[k]) can meet; false when the
+// segments' union spans more than two grid cells along some axis.
+RSX_HD bool ray4_block(const MeshView& w, const V3* mn, const V3* mx, int& blk) {
     const V3 umn = vmin(vmin(mn[0], mn[1]), vmin(mn[2], mn[3]));
     const V3 umx = vmax(vmax(mx[0], mx[1]), vmax(mx[2], mx[3]));
     const float inv = 1.0f / w.grid_cell;
@@ -281,7 +278,25 @@ RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* h
     const int x1 = clampi((int)floorf((umx.x - w.grid_min.x) * inv), 0, w.gx - 1);
     const int y1 = clampi((int)floorf((umx.y - w.grid_min.y) * inv), 0, w.gy - 1);
     const int z1 = clampi((int)floorf((umx.z - w.grid_min.z) * inv), 0, w.gz - 1);
-    if (x1 - x0 > 1 || y1 - y0 > 1 || z1 - z0 > 1) {
+    blk = (x0 * w.gy + y0) * w.gz + z0;
+    return x1 - x0 <= 1 && y1 - y0 <= 1 && z1 - z0 <= 1;
+}
+
+// Whether a grid entry's AABB meets a segment's AABB (mn, mx).
+RSX_HD bool entry_meets_segment(const GridEntry& g, V3 mn, V3 mx) {
+    return !(g.mn[2] > mx.z || g.mx[2] < mn.z) && !(g.mn[0] > mx.x || g.mx[0] < mn.x) &&
+           !(g.mn[1] > mx.y || g.mx[1] < mn.y);
+}
+
+RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* hit) {
+    V3 mn[4], mx[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        mn[k] = vmin(from[k], to[k]);
+        mx[k] = vmax(from[k], to[k]);
+    }
+    int blk;
+    if (!ray4_block(w, mn, mx, blk)) {
         for (int k = 0; k < 4; k++) ray_mesh(w, from[k], to[k], hit[k]);
         return;
     }
@@ -292,7 +307,6 @@ RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* h
         frac[k] = hit[k].object >= 0 ? hit[k].fraction : 1.0f;
         best[k] = -1;
     }
-    const int blk = (x0 * w.gy + y0) * w.gz + z0;
     const int s = w.blk_start[blk], e = w.blk_start[blk + 1];
     for (int j0 = s; j0 < e; j0 += 2) {
         // two entries per round, their loads issued together
@@ -308,8 +322,7 @@ RSX_HD void ray_mesh4(const MeshView& w, const V3* from, const V3* to, RayHit* h
         bool any = false;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            ov[k] = !(g.mn[2] > mx[k].z || g.mx[2] < mn[k].z) && !(g.mn[0] > mx[k].x || g.mx[0] < mn[k].x) &&
-                    !(g.mn[1] > mx[k].y || g.mx[1] < mn[k].y);
+            ov[k] = entry_meets_segment(g, mn[k], mx[k]);
             any = any || ov[k];
         }
         if (!any) continue;

@@ -339,6 +339,130 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
     sdf::wave_sync();
   }
 }
+
+// LDS of a wavefront's pooled wheel-ray tests, in the pool's space (free
+// during the pre-tick): each lane's four ray segments, its block's first
+// entry and entry count, the lanes' exclusive prefix counts, and per ray the
+// smallest (fraction bits, packed key) found so far.
+constexpr int kRayPoolSegOff = 0;
+constexpr int kRayPoolFirstOff = kRayPoolSegOff + 64 * 8 * (int)sizeof(V3);
+constexpr int kRayPoolCntOff = kRayPoolFirstOff + 64 * 4;
+constexpr int kRayPoolPreOff = kRayPoolCntOff + 64 * 4;
+constexpr int kRayPoolBestOff = kRayPoolPreOff + 64 * 4;
+constexpr int kRayPoolBytes = kRayPoolBestOff + 64 * 4 * 8;
+static_assert(kRayPoolBestOff % 8 == 0 && kRayPoolBytes <= kPoolBytes, "ray pool layout");
+
+// The mesh part of car `ci`'s four suspension rays on every lane of the
+// wavefront (ray_mesh4's results), spread over its 64 lanes: each lane puts
+// its rays and its block of grid entries in LDS, and the wavefront's entries
+// are tested 64 at a time, each against every ray of its owner whose box it
+// meets.  A ray's hit is the smallest (fraction, packed key) among the
+// triangles it crosses below fraction 1; a crossing fraction is da / (da - db)
+// with da and db of opposite signs, never negative, so its bits order as the
+// float does and one 64-bit minimum of (fraction bits, key) keeps it.  The
+// owner then recomputes that triangle's crossing for the normal.  A lane
+// whose rays' union spans more than two cells per axis walks the grid per ray
+// afterwards, as ray_mesh4 does.  `want` false: the lane casts nothing (hit
+// stays object -1).  Every lane of the wavefront must call it.
+template <int MAXC>
+__device__ void car_mesh_rays_pool(const Arena<MAXC>& A, const TickCtx& T, int ci, bool want, RayHit* hit) {
+  if constexpr (MAXC > 0) {
+    const MeshView& w = T.mv;
+    const int l = threadIdx.x & 63;
+    V3* seg = (V3*)(T.pool + kRayPoolSegOff);
+    int* first = (int*)(T.pool + kRayPoolFirstOff);
+    int* cnt = (int*)(T.pool + kRayPoolCntOff);
+    int* pre = (int*)(T.pool + kRayPoolPreOff);
+    unsigned long long* best = (unsigned long long*)(T.pool + kRayPoolBestOff);
+    V3 from[4], to[4];
+    for (int k = 0; k < 4; k++) {
+        hit[k].object = -1;
+        hit[k].fraction = 1.0f;
+    }
+    bool walk = false;
+    int s = 0, n = 0;
+    if (want) {
+        V3 mn[4], mx[4];
+        for (int k = 0; k < 4; k++) {
+            wheel_ray_segment(A.car[ci], T.cg, k, from[k], to[k]);
+            mn[k] = vmin(from[k], to[k]);
+            mx[k] = vmax(from[k], to[k]);
+        }
+        int blk;
+        if (ray4_block(w, mn, mx, blk)) {
+            s = w.blk_start[blk];
+            n = w.blk_start[blk + 1] - s;
+        } else {
+            walk = true;
+        }
+    }
+    int incl = n;  // inclusive prefix of the entry counts over the lanes
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (l >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    sdf::wave_sync();  // the previous readers of the pool are done
+    if (n) {
+        for (int k = 0; k < 4; k++) {
+            seg[l * 8 + 2 * k] = from[k];
+            seg[l * 8 + 2 * k + 1] = to[k];
+        }
+    }
+    for (int k = 0; k < 4; k++) best[l * 4 + k] = ~0ull;
+    first[l] = s;
+    cnt[l] = n;
+    pre[l] = incl - n;
+    sdf::wave_sync();
+    for (int t0 = 0; t0 < total; t0 += 64) {
+        const int t = t0 + l;
+        if (t >= total) continue;
+        // the owner: the last lane whose exclusive prefix is <= t (it has entries)
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (pre[j + step] <= t) j += step;
+        const GridEntry g = w.blk_ent[first[j] + (t - pre[j])];
+        V3 f[4], e[4];
+        bool ov[4];
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            f[k] = seg[j * 8 + 2 * k];
+            e[k] = seg[j * 8 + 2 * k + 1];
+            ov[k] = entry_meets_segment(g, vmin(f[k], e[k]), vmax(f[k], e[k]));
+            any = any || ov[k];
+        }
+        if (!any) continue;
+        V3 a, b, c;
+        load_tri(w, cand_tri(g.key), a, b, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!ov[k]) continue;
+            V3 nrm;
+            float d;
+            if (!ray_triangle_hit(f[k], e[k], a, b, c, d, nrm) || !(d < 1.0f)) continue;
+            atomicMin(&best[j * 4 + k], ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)g.key);
+        }
+    }
+    sdf::wave_sync();
+    if (n) {
+        for (int k = 0; k < 4; k++) {
+            const unsigned long long v = best[l * 4 + k];
+            if (v == ~0ull) continue;
+            V3 a, b, c, nrm;
+            float d;
+            load_tri(w, cand_tri((int)(unsigned)v), a, b, c);
+            ray_triangle_hit(from[k], to[k], a, b, c, d, nrm);
+            hit[k].fraction = d;
+            hit[k].normal = nrm;
+            hit[k].object = 0;
+        }
+    }
+    if (walk)
+        for (int k = 0; k < 4; k++) ray_mesh(w, from[k], to[k], hit[k]);
+  }
+}
 #endif
 
 // Narrowphase of car `ci`'s box against mesh `mesh`
@@ -975,6 +1099,17 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
             }
             c.active = false;  // body disabled for this tick
         }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (T.pool) {
+            // the mesh part of this car slot's wheel rays on every lane,
+            // spread over the wavefront (a demolished car casts none)
+            RayHit cast[4];
+            car_mesh_rays_pool(A, T, i, !c.is_demoed && !far[1 + i], cast);
+            if (c.is_demoed) continue;
+            car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i], cast);
+            continue;
+        }
+#endif
         if (c.is_demoed) continue;
         car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i]);
     }
