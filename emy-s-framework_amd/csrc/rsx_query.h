@@ -266,11 +266,10 @@ RSX_HD void ray_mesh(const MeshView& w, V3 from, V3 to, RayHit& hit) {
 // same hits; a car over an empty block costs one lookup instead of four grid
 // walks.  hit[k] holds what ray k hit before (or object -1).
 // The 2 x 2 x 2 block (the grid cell at its lowest corner) listing every
-// triangle four ray segments (boxes mn[k], mx[k]) can meet; false when the
-// segments' union spans more than two grid cells along some axis.
-RSX_HD bool ray4_block(const MeshView& w, const V3* mn, const V3* mx, int& blk) {
-    const V3 umn = vmin(vmin(mn[0], mn[1]), vmin(mn[2], mn[3]));
-    const V3 umx = vmax(vmax(mx[0], mx[1]), vmax(mx[2], mx[3]));
+// triangle whose AABB can meet the box [mn, mx] (gather_tris's cell
+// arithmetic); false when the box spans more than two grid cells along some
+// axis.
+RSX_HD bool box_block(const MeshView& w, V3 umn, V3 umx, int& blk) {
     const float inv = 1.0f / w.grid_cell;
     const int x0 = clampi((int)floorf((umn.x - w.grid_min.x) * inv), 0, w.gx - 1);
     const int y0 = clampi((int)floorf((umn.y - w.grid_min.y) * inv), 0, w.gy - 1);
@@ -280,6 +279,13 @@ RSX_HD bool ray4_block(const MeshView& w, const V3* mn, const V3* mx, int& blk) 
     const int z1 = clampi((int)floorf((umx.z - w.grid_min.z) * inv), 0, w.gz - 1);
     blk = (x0 * w.gy + y0) * w.gz + z0;
     return x1 - x0 <= 1 && y1 - y0 <= 1 && z1 - z0 <= 1;
+}
+
+// The block for four ray segments (boxes mn[k], mx[k]): that of their union.
+RSX_HD bool ray4_block(const MeshView& w, const V3* mn, const V3* mx, int& blk) {
+    const V3 umn = vmin(vmin(mn[0], mn[1]), vmin(mn[2], mn[3]));
+    const V3 umx = vmax(vmax(mx[0], mx[1]), vmax(mx[2], mx[3]));
+    return box_block(w, umn, umx, blk);
 }
 
 // Whether a grid entry's AABB meets a segment's AABB (mn, mx).

@@ -70,9 +70,9 @@ struct TickScratch {
     MfList<MAXC> L;
     Solver<MAXC> S;
     V3 bmin[MAXC + 1], bmax[MAXC + 1];   // broadphase AABBs of dynamic bodies
-    int cand[kMaxCand];
-    int16_t pidx[kMaxCand];              // candidate -> index among the lane's pooled queries, -1: separated
+    int cand[kMaxCand];                  // a body's candidate keys (pooled car queries: the kept ones), key order
     int pool_base;                       // the lane's first slot in the wavefront's pooled results
+    bool pool_local;                     // this car slot's queries run in the lane (its box spans > 2 cells)
     std::conditional_t<(MAXC > 0), gjk::GjkEpa, NoGjk> gj;  // box-vs-triangle GJK / EPA working set
     float cull_pred[MAXC + 1];           // learned-cull distances (uu), -1e30: not usable
     int cull_age;                        // ticks since the network last ran (uniform over the wavefront)
@@ -254,60 +254,148 @@ constexpr int kPoolItemOff = kPoolCntOff + 64 * 4;
 constexpr int kPoolResOff = kPoolItemOff + kPoolCap * 4;
 constexpr int kPoolBytes = kPoolResOff + kPoolCap * (int)sizeof(PooledContact);
 
+// LDS of the gather stage of a wavefront's pooled car queries, laid over the
+// results (written only after it): per lane its block's first entry, entry
+// count and exclusive prefix, its broadphase mesh mask, the meshes its
+// candidates belong to, its box, the inverse of its child basis, and a
+// 256-bit mask over its block's entries that the filter keeps.
+constexpr int kGatFirstOff = kPoolResOff;
+constexpr int kGatCntOff = kGatFirstOff + 64 * 4;
+constexpr int kGatPreOff = kGatCntOff + 64 * 4;
+constexpr int kGatBpOff = kGatPreOff + 64 * 4;
+constexpr int kGatMeshOff = kGatBpOff + 64 * 4;
+constexpr int kGatBoxOff = kGatMeshOff + 64 * 4;
+constexpr int kGatInvOff = kGatBoxOff + 64 * 2 * (int)sizeof(V3);
+constexpr int kGatKeptOff = kGatInvOff + 64 * (int)sizeof(M3);
+constexpr int kGatEnd = kGatKeptOff + 64 * 4 * 8;
+static_assert(kGatKeptOff % 8 == 0 && kGatEnd <= kPoolBytes && kMaxCand <= 256, "car pool gather layout");
+
 #if defined(__HIP_DEVICE_COMPILE__)
-// Box-triangle queries of car `ci`'s candidates on every lane of the
-// wavefront, spread over its 64 lanes: each lane filters its candidates
-// (meshes outside `mask`, box_tri_separated), the wavefront lists the
-// survivors in LDS, every lane runs entries lane, lane + 64, ... with its own
-// GJK working set, and the results wait in LDS for car_vs_mesh.  A lane's
-// queries otherwise run one after another while the rest of its wavefront
-// idles.  Every lane of the wavefront must call it.
+// Car `ci`'s mesh queries on every lane of the wavefront, spread over its 64
+// lanes: the candidate gather (gather_tris's block scan under the car box),
+// the filter (meshes outside the broadphase mask `bp`, box_tri_separated)
+// and the box-triangle queries (gjk::closest_points, then the contact's
+// internal-edge correction).  Each lane puts its block range, box and child
+// transform in LDS; the wavefront tests the block entries 64 at a time,
+// marking in the owner's masks the meshes of its candidates and the entries
+// the filter keeps; each owner lists its kept triangles in key order
+// (W.cand, `ncand` of them) and as query items; every lane then runs items
+// lane, lane + 64, ... with its own GJK working set, and the results wait in
+// LDS for car_vs_mesh.  One lane's gather and queries would otherwise run one
+// after another while the rest of its wavefront idles.  Returns the meshes
+// with candidates.  A lane whose box spans more than two grid cells along
+// some axis gathers with gather_tris and runs its queries in car_vs_mesh
+// (W.pool_local).  `want` false: no candidates.  Every lane of the wavefront
+// must call it.
 template <int MAXC>
-__device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int ci, uint32_t mask,
-                             int ncand) {
+__device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int ci, bool want,
+                                  uint32_t bp, int& ncand) {
+    uint32_t cmask = 0;
+    ncand = 0;
+    W.pool_local = false;
   if constexpr (MAXC > 0) {
+    const MeshView& w = T.mv;
     const int l = threadIdx.x & 63;
     const CarGeom& g = T.cg;
     Xf* xfs = (Xf*)(T.pool + kPoolXfOff);
-    int* cnt = (int*)(T.pool + kPoolCntOff);
     uint32_t* items = (uint32_t*)(T.pool + kPoolItemOff);
     PooledContact* res = (PooledContact*)(T.pool + kPoolResOff);
+    int* gfirst = (int*)(T.pool + kGatFirstOff);
+    int* gcnt = (int*)(T.pool + kGatCntOff);
+    int* gpre = (int*)(T.pool + kGatPreOff);
+    uint32_t* gbp = (uint32_t*)(T.pool + kGatBpOff);
+    uint32_t* gmesh = (uint32_t*)(T.pool + kGatMeshOff);
+    V3* gbox = (V3*)(T.pool + kGatBoxOff);
+    M3* ginv = (M3*)(T.pool + kGatInvOff);
+    unsigned long long* gkept = (unsigned long long*)(T.pool + kGatKeptOff);
     Xf child = xf_identity();
-    int n_items = 0;
-    if (mask) {
+    V3 cmn = V3{0, 0, 0}, cmx = V3{0, 0, 0};
+    int s = 0, n = 0;
+    if (want) {
         child = car_child_xf(g, A.car[ci].xf);
-        const M3 inv_basis = inverse(child.b);
-        const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
-        for (int i = 0; i < ncand; i++) {
-            V3 tri[3];
-            load_tri(T.mv, cand_tri(W.cand[i]), tri[0], tri[1], tri[2]);
-            const bool keep = ((mask >> cand_mesh(W.cand[i])) & 1u) &&
-                              !box_tri_separated(child, inv_basis, hm, tri, g.breaking);
-            W.pidx[i] = keep ? (int16_t)n_items++ : (int16_t)-1;
+        box_child_aabb(g, child, cmn, cmx);
+        int blk;
+        if (box_block(w, cmn, cmx, blk)) {
+            s = w.blk_start[blk];
+            n = w.blk_start[blk + 1] - s;
+        } else {
+            W.pool_local = true;
+            ncand = gather_tris(w, cmn, cmx, W.cand, kMaxCand);
+            for (int i = 0; i < ncand; i++) cmask |= 1u << cand_mesh(W.cand[i]);
         }
     }
+    int incl = n;  // inclusive prefix of the entry counts over the lanes
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (l >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
     sdf::wave_sync();  // the previous readers of the pool are done
     xfs[l] = child;
-    cnt[l] = n_items;
+    gfirst[l] = s;
+    gcnt[l] = n;
+    gpre[l] = incl - n;
+    gbp[l] = bp;
+    gmesh[l] = 0;
+    gbox[2 * l] = cmn;
+    gbox[2 * l + 1] = cmx;
+    if (n) ginv[l] = inverse(child.b);
+    for (int k = 0; k < 4; k++) gkept[l * 4 + k] = 0;
     sdf::wave_sync();
-    int base = 0, total = 0;
-    for (int j = 0; j < 64; j++) {
-        const int c = cnt[j];
-        base += j < l ? c : 0;
-        total += c;
+    const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
+    for (int t0 = 0; t0 < total; t0 += 64) {
+        const int t = t0 + l;
+        if (t >= total) continue;
+        // the owner: the last lane whose exclusive prefix is <= t (it has entries)
+        int j = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (gpre[j + step] <= t) j += step;
+        const int off = t - gpre[j];
+        const GridEntry e = w.blk_ent[gfirst[j] + off];
+        const V3 mn = gbox[2 * j], mx = gbox[2 * j + 1];
+        // gather_tris's AABB test: z, then x, then y
+        if (e.mn[2] > mx.z || e.mx[2] < mn.z) continue;
+        if (e.mn[0] > mx.x || e.mx[0] < mn.x) continue;
+        if (e.mn[1] > mx.y || e.mx[1] < mn.y) continue;
+        const int mesh = cand_mesh(e.key);
+        atomicOr(&gmesh[j], 1u << mesh);
+        if (!((gbp[j] >> mesh) & 1u)) continue;
+        V3 tri[3];
+        load_tri(w, cand_tri(e.key), tri[0], tri[1], tri[2]);
+        if (box_tri_separated(xfs[j], ginv[j], hm, tri, g.breaking)) continue;
+        atomicOr(&gkept[j * 4 + (off >> 6)], 1ull << (off & 63));
     }
-    W.pool_base = base;
-    if (mask)
-        for (int i = 0; i < ncand; i++) {
-            const int it = base + W.pidx[i];
-            if (W.pidx[i] >= 0 && it < kPoolCap) items[it] = ((uint32_t)l << 16) | (uint32_t)cand_tri(W.cand[i]);
-        }
     sdf::wave_sync();
-    const int n = total < kPoolCap ? total : kPoolCap;
+    if (n) {
+        // the kept entries in block order, which is key order
+        cmask = gmesh[l];
+        for (int k = 0; k < 4; k++) {
+            unsigned long long b = gkept[l * 4 + k];
+            while (b) {
+                const int bit = __builtin_ctzll(b);
+                b &= b - 1;
+                W.cand[ncand++] = w.blk_ent[s + k * 64 + bit].key;
+            }
+        }
+    }
+    const int kept = W.pool_local ? 0 : ncand;
+    int kincl = kept;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(kincl, o);
+        if (l >= o) kincl += v;
+    }
+    const int ktotal = __shfl(kincl, 63);
+    const int base = kincl - kept;
+    W.pool_base = base;
+    for (int r = 0; r < kept && base + r < kPoolCap; r++)
+        items[base + r] = ((uint32_t)l << 16) | (uint32_t)cand_tri(W.cand[r]);
+    sdf::wave_sync();
+    const int nq = ktotal < kPoolCap ? ktotal : kPoolCap;
     const float max_d = g.box_margin + g.breaking;
-    for (int it0 = 0; it0 < n; it0 += 64) {
+    for (int it0 = 0; it0 < nq; it0 += 64) {
         const int it = it0 + l;
-        if (it < n) {
+        if (it < nq) {
             const uint32_t e = items[it];
             gjk::Shapes sh;
             load_tri(T.mv, (int)(e & 0xffffu), sh.tri[0], sh.tri[1], sh.tri[2]);
@@ -338,6 +426,7 @@ __device__ void car_gjk_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch
     }
     sdf::wave_sync();
   }
+    return cmask;
 }
 
 // LDS of a wavefront's pooled wheel-ray tests, in the pool's space (free
@@ -469,8 +558,8 @@ __device__ void car_mesh_rays_pool(const Arena<MAXC>& A, const TickCtx& T, int c
 // (btConvexConcaveCollisionAlgorithm with the box as the convex body: the
 // triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
 // one manifold per car and mesh, refreshed after all triangles).  With
-// pooling the queries ran in car_gjk_pool and only their results are read
-// here, in the same order.
+// pooling the candidates are the triangles the filter kept, whose queries
+// ran in car_mesh_pool: only their results are read here, in the same order.
 template <int MAXC>
 RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
                         int ncand, int& pos) {
@@ -484,7 +573,8 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
     const M3 inv_basis = inverse(child.b);
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
     const float max_d = g.box_margin + 0.f + m.breaking;
-    const PooledContact* pooled = T.pool ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
+    const PooledContact* pooled =
+        T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
     int i = pos;
     for (; i < ncand; i++) {
         const int pk = cand[i];
@@ -492,9 +582,8 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
         if (pooled) {
-            // the wavefront ran this lane's queries already (car_gjk_pool)
-            if (W.pidx[i] < 0) continue;
-            const int it = W.pool_base + W.pidx[i];
+            // the wavefront ran this lane's queries already (car_mesh_pool)
+            const int it = W.pool_base + i;
             if (it < kPoolCap) {
                 const PooledContact& r = pooled[it];
                 const int hit = r.hit;
@@ -627,7 +716,18 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         // per body; a culled body opens its manifolds and skips the rest.
         const uint32_t open = demoed ? 0u : mask;
         int ncand = 0;
-        if (!demoed && mask && active[d] && !far[d]) {
+        const bool want = !demoed && mask && active[d] && !far[d];
+        bool pooled_car = false;
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (T.pool && d >= 1) {
+            // the car's gather, filter and box-triangle queries, pooled over
+            // the wavefront (every lane calls it)
+            pooled_car = true;
+            const uint32_t cmask = car_mesh_pool(A, T, W, d - 1, want, mask, ncand);
+            mask = want ? (mask & cmask) : 0u;
+        }
+#endif
+        if (!pooled_car && want) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
                 ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, kMaxCand);
@@ -640,12 +740,9 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             uint32_t cmask = 0;
             for (int i = 0; i < ncand; i++) cmask |= 1u << cand_mesh(W.cand[i]);
             mask &= cmask;
-        } else {
+        } else if (!pooled_car) {
             mask = 0;
         }
-#if defined(__HIP_DEVICE_COMPILE__)
-        if (T.pool && d >= 1) car_gjk_pool(A, T, W, d - 1, mask, ncand);
-#endif
         if (demoed) continue;  // no contact response: never solved
         int cpos = 0;  // the candidates not yet consumed by an earlier mesh
         for (int ms = 0; ms < kNumMeshes; ms++) {
