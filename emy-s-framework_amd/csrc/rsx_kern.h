@@ -53,7 +53,7 @@ struct WorldArgs {
     MeshView mv;
     const PadGeom* pads;
     sdf::Cull cull;
-    int pool;                  // pool each wavefront's car mesh queries and wheel rays (car_mesh_pool, car_mesh_rays_pool)
+    int pool;                  // pool each wavefront's car mesh queries and wheel rays (body_mesh_pool, car_mesh_rays_pool)
     int lds_bytes;             // each wavefront's LDS share (arena_lds_per_wave)
 };
 

@@ -271,25 +271,28 @@ constexpr int kGatEnd = kGatKeptOff + 64 * 4 * 8;
 static_assert(kGatKeptOff % 8 == 0 && kGatEnd <= kPoolBytes && kMaxCand <= 256, "car pool gather layout");
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// Car `ci`'s mesh queries on every lane of the wavefront, spread over its 64
-// lanes: the candidate gather (gather_tris's block scan under the car box),
-// the filter (meshes outside the broadphase mask `bp`, box_tri_separated)
-// and the box-triangle queries (gjk::closest_points, then the contact's
-// internal-edge correction).  Each lane puts its block range, box and child
-// transform in LDS; the wavefront tests the block entries 64 at a time,
-// marking in the owner's masks the meshes of its candidates and the entries
-// the filter keeps; each owner lists its kept triangles in key order
-// (W.cand, `ncand` of them) and as query items; every lane then runs items
-// lane, lane + 64, ... with its own GJK working set, and the results wait in
-// LDS for car_vs_mesh.  One lane's gather and queries would otherwise run one
-// after another while the rest of its wavefront idles.  Returns the meshes
-// with candidates.  A lane whose box spans more than two grid cells along
-// some axis gathers with gather_tris and runs its queries in car_vs_mesh
-// (W.pool_local).  `want` false: no candidates.  Every lane of the wavefront
-// must call it.
+// Body `d`'s mesh queries (0 the ball, 1 + i car i) on every lane of the
+// wavefront, spread over its 64 lanes: the candidate gather (gather_tris's
+// block scan under the body's box) and the filter (meshes outside the
+// broadphase mask `bp`; for a car box_tri_separated, for the ball the
+// sphere-triangle early-out and test of ball_vs_mesh), then for a car the
+// box-triangle queries (gjk::closest_points, then the contact's internal-edge
+// correction).  Each lane puts its block range, box and transform in LDS;
+// the wavefront tests the block entries 64 at a time, marking in the owner's
+// masks the meshes of its candidates and the entries the filter keeps; each
+// owner lists its kept triangles in key order (W.cand, `ncand` of them) --
+// for a car also as query items, which every lane then runs lane, lane + 64,
+// ... with its own GJK working set, the results waiting in LDS for
+// car_vs_mesh.  The ball's kept triangles are exactly those for which
+// ball_vs_mesh adds a point, which it then does over this shorter list.  One
+// lane's gather and tests would otherwise run one after another while the
+// rest of its wavefront idles.  Returns the meshes with candidates.  A lane
+// whose box spans more than two grid cells along some axis gathers with
+// gather_tris and filters in its narrowphase (W.pool_local).  `want` false:
+// no candidates.  Every lane of the wavefront must call it, with the same d.
 template <int MAXC>
-__device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int ci, bool want,
-                                  uint32_t bp, int& ncand) {
+__device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int d, bool want,
+                                   uint32_t bp, int& ncand) {
     uint32_t cmask = 0;
     ncand = 0;
     W.pool_local = false;
@@ -311,9 +314,16 @@ __device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickSc
     Xf child = xf_identity();
     V3 cmn = V3{0, 0, 0}, cmx = V3{0, 0, 0};
     int s = 0, n = 0;
-    if (want) {
-        child = car_child_xf(g, A.car[ci].xf);
+    if (want && d == 0) {
+        const V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
+        child.o = A.ball.xf.o;
+        cmn = A.ball.xf.o - r;
+        cmx = A.ball.xf.o + r;
+    } else if (want) {
+        child = car_child_xf(g, A.car[d - 1].xf);
         box_child_aabb(g, child, cmn, cmx);
+    }
+    if (want) {
         int blk;
         if (box_block(w, cmn, cmx, blk)) {
             s = w.blk_start[blk];
@@ -339,7 +349,7 @@ __device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickSc
     gmesh[l] = 0;
     gbox[2 * l] = cmn;
     gbox[2 * l + 1] = cmx;
-    if (n) ginv[l] = inverse(child.b);
+    if (n && d >= 1) ginv[l] = inverse(child.b);
     for (int k = 0; k < 4; k++) gkept[l * 4 + k] = 0;
     sdf::wave_sync();
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
@@ -363,7 +373,15 @@ __device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickSc
         if (!((gbp[j] >> mesh) & 1u)) continue;
         V3 tri[3];
         load_tri(w, cand_tri(e.key), tri[0], tri[1], tri[2]);
-        if (box_tri_separated(xfs[j], ginv[j], hm, tri, g.breaking)) continue;
+        if (d == 0) {
+            const V3 c = xfs[j].o;
+            if (sphere_tri_early_out(c, T.bg.radius, tri[0], tri[1], tri[2], T.bg.breaking)) continue;
+            V3 pt, nrm;
+            float depth;
+            if (!sphere_triangle(c, T.bg.radius, T.bg.breaking, tri[0], tri[1], tri[2], pt, nrm, depth)) continue;
+        } else if (box_tri_separated(xfs[j], ginv[j], hm, tri, g.breaking)) {
+            continue;
+        }
         atomicOr(&gkept[j * 4 + (off >> 6)], 1ull << (off & 63));
     }
     sdf::wave_sync();
@@ -379,6 +397,7 @@ __device__ uint32_t car_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickSc
             }
         }
     }
+    if (d == 0) return cmask;  // the ball: no queries to pool
     const int kept = W.pool_local ? 0 : ncand;
     int kincl = kept;
     for (int o = 1; o < 64; o <<= 1) {
@@ -559,7 +578,7 @@ __device__ void car_mesh_rays_pool(const Arena<MAXC>& A, const TickCtx& T, int c
 // triangle early-out, then btConvexConvexAlgorithm's GJK / EPA per triangle;
 // one manifold per car and mesh, refreshed after all triangles).  With
 // pooling the candidates are the triangles the filter kept, whose queries
-// ran in car_mesh_pool: only their results are read here, in the same order.
+// ran in body_mesh_pool: only their results are read here, in the same order.
 template <int MAXC>
 RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, Manifold& m, int ci, int mesh,
                         int ncand, int& pos) {
@@ -582,7 +601,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
         if (pooled) {
-            // the wavefront ran this lane's queries already (car_mesh_pool)
+            // the wavefront ran this lane's queries already (body_mesh_pool)
             const int it = W.pool_base + i;
             if (it < kPoolCap) {
                 const PooledContact& r = pooled[it];
@@ -717,17 +736,17 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         const uint32_t open = demoed ? 0u : mask;
         int ncand = 0;
         const bool want = !demoed && mask && active[d] && !far[d];
-        bool pooled_car = false;
+        bool pooled_body = false;
 #if defined(__HIP_DEVICE_COMPILE__)
-        if (T.pool && d >= 1) {
-            // the car's gather, filter and box-triangle queries, pooled over
-            // the wavefront (every lane calls it)
-            pooled_car = true;
-            const uint32_t cmask = car_mesh_pool(A, T, W, d - 1, want, mask, ncand);
+        if (T.pool) {
+            // the body's gather, filter and (cars) box-triangle queries,
+            // pooled over the wavefront (every lane calls it)
+            pooled_body = true;
+            const uint32_t cmask = body_mesh_pool(A, T, W, d, want, mask, ncand);
             mask = want ? (mask & cmask) : 0u;
         }
 #endif
-        if (!pooled_car && want) {
+        if (!pooled_body && want) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
                 ncand = gather_tris(T.mv, A.ball.xf.o - r, A.ball.xf.o + r, W.cand, kMaxCand);
@@ -740,7 +759,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             uint32_t cmask = 0;
             for (int i = 0; i < ncand; i++) cmask |= 1u << cand_mesh(W.cand[i]);
             mask &= cmask;
-        } else if (!pooled_car) {
+        } else if (!pooled_body) {
             mask = 0;
         }
         if (demoed) continue;  // no contact response: never solved
