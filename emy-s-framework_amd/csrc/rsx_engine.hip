@@ -68,7 +68,7 @@ struct Bucket {
     int* d_global = nullptr;         // n: global arena index
     std::vector<int> global;         // bucket-local -> global arena index
     uint32_t* d_vanta = nullptr;     // VantaObs state words (field-major), allocated with the first Vanta config
-    int* d_vwork = nullptr;          // n + 1 ints: the rollouts' arena list and its length
+    int* d_vwork = nullptr;          // 2n + 2 ints: the rollouts' two arena lists and their lengths
 };
 
 struct rsx_engine {
@@ -322,7 +322,7 @@ hipError_t vanta_launch(rsx_engine* e, const uint8_t* reset, hipStream_t st) {
     v.obs = e->d_obs;
     for (const Bucket& b : e->bucket) {
         if (!b.n) continue;
-        const hipError_t r = launch_vanta(b.maxc, bucket_args(b), b.d_vanta, reset, v, b.d_vwork, b.d_vwork + b.n, st);
+        const hipError_t r = launch_vanta(b.maxc, bucket_args(b), b.d_vanta, reset, v, b.d_vwork, st);
         if (r != hipSuccess) return r;
     }
     return hipSuccess;
@@ -370,8 +370,8 @@ rsx_status vanta_alloc(rsx_engine* e) {
         const size_t bytes = sizeof(uint32_t) * (size_t)vanta_words(b.maxc) * b.n;
         HIP_TRY(hipMalloc((void**)&b.d_vanta, bytes));
         HIP_TRY(hipMemset(b.d_vanta, 0, bytes));
-        HIP_TRY(hipMalloc((void**)&b.d_vwork, sizeof(int) * ((size_t)b.n + 1)));
-        HIP_TRY(hipMemset(b.d_vwork, 0, sizeof(int) * ((size_t)b.n + 1)));
+        HIP_TRY(hipMalloc((void**)&b.d_vwork, sizeof(int) * (2 * (size_t)b.n + 2)));
+        HIP_TRY(hipMemset(b.d_vwork, 0, sizeof(int) * (2 * (size_t)b.n + 2)));
     }
     e->vanta_alloc = true;
     return RSX_OK;

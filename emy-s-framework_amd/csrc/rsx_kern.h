@@ -111,9 +111,9 @@ struct VantaArgs {
     vanta::Params Q;
     float* obs;
 };
-// work (b.n ints) and count (one int) are the rollouts' arena list.
+// lists (2 b.n + 2 ints): the rollouts' arena lists and their lengths.
 hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const uint8_t* reset, const VantaArgs& v,
-                        int* work, int* count, hipStream_t st);
+                        int* lists, hipStream_t st);
 // Clears the Vanta words of every arena of a bucket (only < 0) or of arena `only`.
 hipError_t launch_vanta_clear(int maxc, uint32_t* vwords, int n, int only, hipStream_t st);
 

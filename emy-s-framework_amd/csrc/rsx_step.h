@@ -296,7 +296,6 @@ __device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickS
     uint32_t cmask = 0;
     ncand = 0;
     W.pool_local = false;
-  if constexpr (MAXC > 0) {
     const MeshView& w = T.mv;
     const int l = threadIdx.x & 63;
     const CarGeom& g = T.cg;
@@ -398,6 +397,7 @@ __device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickS
         }
     }
     if (d == 0) return cmask;  // the ball: no queries to pool
+  if constexpr (MAXC > 0) {
     const int kept = W.pool_local ? 0 : ncand;
     int kincl = kept;
     for (int o = 1; o < 64; o <<= 1) {

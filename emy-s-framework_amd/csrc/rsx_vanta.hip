@@ -30,15 +30,17 @@ __device__ inline void ball_only_ctx(TickCtx& T, const VantaArgs& V, gjk::Simple
 }
 
 // The ball-prediction rollouts (vantaArenaUpdateKernel's first half,
-// RocketSimCudaObs.cuh:92-121), in two passes so that no wavefront runs the
-// full 360 ticks for one lane while its others idle.  k_vanta_pred_step, one
-// lane per arena: clears the Vanta words of an arena that was reset; keeps the
-// slices when neither the tick nor the ball moved since the last rollout;
-// when the ball is, bit for bit, the one the last rollout passed ts ticks
-// after its start, continues that rollout (4 x ts ticks, rollout_continue:
-// the same slices as a new rollout); otherwise lists the arena for
-// k_vanta_pred_full, which runs the full rollouts of the listed arenas, one
-// per lane, in dense wavefronts.  Both read the ball from the arena words.
+// RocketSimCudaObs.cuh:92-121), in passes so that no wavefront runs one
+// lane's rollout while its others idle.  k_vanta_pred_step, one lane per
+// arena: clears the Vanta words of an arena that was reset; keeps the slices
+// when neither the tick nor the ball moved since the last rollout; when the
+// ball is, bit for bit, the one the last rollout passed ts ticks after its
+// start, lists the arena for k_vanta_pred_cont, which continues that rollout
+// (rollout_continue: the same slices as a new rollout) with its four chain
+// states on four lanes; otherwise lists it for k_vanta_pred_full, which runs
+// the full rollouts, one per lane.  Both run in dense wavefronts and read the
+// ball from the arena words.  `lists`: 2n + 2 ints, the new-rollout list
+// [0, n), the continue list [n, 2n), then their two lengths.
 __device__ inline vanta::BallQ arena_ball_q(const Col& col, int ball_base) {
     return vanta::BallQ{col.getv(ball_base + 0), col.getv(ball_base + 3), col.getv(ball_base + 6)};
 }
@@ -58,12 +60,12 @@ __device__ void store_rollout(const Col& vc, uint64_t tick, const vanta::BallQ& 
 }
 
 __global__ void k_vanta_count_zero(int* count) {
-    if (threadIdx.x == 0) *count = 0;
+    if (threadIdx.x < 2) count[threadIdx.x] = 0;
 }
 
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_vanta_pred_step(BucketArgs B, uint32_t* vwords, const uint8_t* reset,
-                                                        VantaArgs V, int* work, int* count) {
+                                                        VantaArgs V, int* fresh, int* cont, int* count) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= B.n) return;
     const Col col{B.words + a, B.n};
@@ -77,31 +79,80 @@ __global__ void __launch_bounds__(64) k_vanta_pred_step(BucketArgs B, uint32_t* 
     if (chains && tick == t0 && vanta::same_ball(now, vanta::get_q(vc, vanta::kVaStart))) return;  // nothing moved
     if (chains && vanta::chains_fit(ts) && tick == t0 + (uint64_t)ts &&
         vanta::same_ball(now, vanta::get_q(vc, vanta::kVaChain))) {
-        vanta::BallQ chain[4];
-        for (int k = 0; k < 4; k++) chain[k] = vanta::get_q(vc, vanta::kVaChain + 9 * k);
-        gjk::Simplex sx;
-        TickCtx T;
-        ball_only_ctx(T, V, &sx);
-        TickScratch<0> W;
-        W.cull_age = 1 << 30;
-        W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
-        W.L.overflow = false;
-        Arena<0> P;
-        vanta::Roller R{T, W, P};
-        V3 pos[3], vel[3];
-        vanta::rollout_continue(R, ts, pos, vel, chain);
-        store_rollout<MAXC>(vc, tick, now, pos, vel, chain, true);
+        cont[atomicAdd(&count[1], 1)] = a;
         return;
     }
-    work[atomicAdd(count, 1)] = a;
+    fresh[atomicAdd(&count[0], 1)] = a;
 }
 
+// A ball-only arena's tick context with the pooled ball gather in `pool`.
+__device__ inline void pooled_ball_ctx(TickCtx& T, const VantaArgs& V, gjk::Simplex* sx, uint8_t* pool, bool own) {
+    ball_only_ctx(T, V, sx);
+    T.pool = pool;
+    T.own = own;
+}
+
+// The continued rollouts (rollout_continue), four lanes per listed arena:
+// lane k advances the rollout's chain state k (its state at ts, 120 + ts,
+// 240 + ts and 360 ticks) by ts ticks.  Lane 1's input is the first slice
+// (the ball 120 ticks after the new start), lane 2's the second, and lane
+// 3's output the third, so each lane stores what it read or computed.  One
+// wavefront per workgroup; the ball's mesh gather is pooled over its lanes
+// (body_mesh_pool), so lanes past the list's end run a copy of its last lane
+// and store nothing.
+template <int MAXC>
+__global__ void __launch_bounds__(64) k_vanta_pred_cont(BucketArgs B, uint32_t* vwords, VantaArgs V, const int* cont,
+                                                        const int* count) {
+    __shared__ __align__(16) uint8_t pool[kGatEnd];
+    const int n = 4 * count[1];
+    if ((int)blockIdx.x * 64 >= n) return;  // the whole wavefront
+    const int i = blockIdx.x * 64 + (int)threadIdx.x;
+    const bool own = i < n;
+    const int ii = own ? i : n - 1;
+    const int a = cont[ii >> 2], k = ii & 3;
+    const Col col{B.words + a, B.n};
+    const Col vc{vwords + a, B.n};
+    const int ts = V.Q.ticks_between;
+    const vanta::BallQ in = vanta::get_q(vc, vanta::kVaChain + 9 * k);
+    gjk::Simplex sx;
+    TickCtx T;
+    pooled_ball_ctx(T, V, &sx, pool, own);
+    TickScratch<0> W;
+    W.cull_age = 1 << 30;
+    W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
+    W.L.overflow = false;
+    Arena<0> P;
+    vanta::Roller R{T, W, P};
+    const vanta::BallQ out = R.advance(in, ts);
+    if (!own) return;
+    vanta::set_q(vc, vanta::kVaChain + 9 * k, out);
+    if (k == 0) {
+        const uint64_t tick = col.get64(Layout<MAXC>::kArenaBase + kArTickLo);
+        vc.set64(vanta::kVaT0, tick);
+        vanta::set_q(vc, vanta::kVaStart, arena_ball_q(col, Layout<MAXC>::kBallBase));
+        vc.set(vanta::kVaAvailHi, vc.get(vanta::kVaAvailHi) | vanta::kVaChainsBit);
+    } else if (k < 3) {
+        vc.setv(vanta::kVaPredPos + 3 * (k - 1), in.p * kBT2UU);
+        vc.setv(vanta::kVaPredVel + 3 * (k - 1), in.v * kBT2UU);
+    } else {
+        vc.setv(vanta::kVaPredPos + 6, out.p * kBT2UU);
+        vc.setv(vanta::kVaPredVel + 6, out.v * kBT2UU);
+    }
+}
+
+// One wavefront per workgroup.  The ball's mesh gather and sphere-triangle
+// tests are pooled over the wavefront's lanes (body_mesh_pool, in the
+// workgroup's static LDS), so every lane runs all 360 ticks: lanes past the
+// list's end run a copy of its last arena and store nothing.
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_vanta_pred_full(BucketArgs B, uint32_t* vwords, VantaArgs V, const int* work,
                                                         const int* count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    const int a = work[i];
+    __shared__ __align__(16) uint8_t pool[kGatEnd];
+    const int n = count[0];
+    if ((int)blockIdx.x * 64 >= n) return;  // the whole wavefront
+    const int i = blockIdx.x * 64 + (int)threadIdx.x;
+    const bool own = i < n;
+    const int a = work[own ? i : n - 1];
     const Col col{B.words + a, B.n};
     const Col vc{vwords + a, B.n};
     const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
@@ -109,7 +160,7 @@ __global__ void __launch_bounds__(64) k_vanta_pred_full(BucketArgs B, uint32_t* 
     const int ts = V.Q.ticks_between;
     gjk::Simplex sx;
     TickCtx T;
-    ball_only_ctx(T, V, &sx);
+    pooled_ball_ctx(T, V, &sx, pool, own);
     TickScratch<0> W;
     W.cull_age = 1 << 30;
     W.cull_counts[0] = W.cull_counts[1] = W.cull_counts[2] = 0;
@@ -120,7 +171,7 @@ __global__ void __launch_bounds__(64) k_vanta_pred_full(BucketArgs B, uint32_t* 
     vanta::BallQ chain[4];
     const bool fit = vanta::chains_fit(ts);
     vanta::rollout_full(R, now, fit ? ts : 0, pos, vel, chain);
-    store_rollout<MAXC>(vc, tick, now, pos, vel, chain, fit);
+    if (own) store_rollout<MAXC>(vc, tick, now, pos, vel, chain, fit);
 }
 
 // The rows: the arena update (pads, demo timers), then every car's coyote
@@ -152,15 +203,20 @@ __global__ void __launch_bounds__(256) k_vanta_clear(uint32_t* vwords, int n, in
 }
 
 hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const uint8_t* reset, const VantaArgs& v,
-                        int* work, int* count, hipStream_t st) {
+                        int* lists, hipStream_t st) {
     if (!b.n) return hipSuccess;
-    dim3 grid((b.n + 63) / 64), block(64);
+    dim3 grid((b.n + 63) / 64), grid4((4 * b.n + 63) / 64), block(64);
+    int* const fresh = lists;
+    int* const cont = lists + b.n;
+    int* const count = lists + 2 * (size_t)b.n;
     if (v.Q.predict) {
         // the rollouts clear the reset arenas' words first
         hipLaunchKernelGGL(k_vanta_count_zero, dim3(1), dim3(64), 0, st, count);
 #define RSX_LAUNCH_PRED(M)                                                                                       \
-    hipLaunchKernelGGL(k_vanta_pred_step<M>, grid, block, 0, st, b, vwords, reset, v, work, count);            \
-    hipLaunchKernelGGL(k_vanta_pred_full<M>, grid, block, 0, st, b, vwords, v, (const int*)work, (const int*)count)
+    hipLaunchKernelGGL(k_vanta_pred_step<M>, grid, block, 0, st, b, vwords, reset, v, fresh, cont, count);     \
+    hipLaunchKernelGGL(k_vanta_pred_cont<M>, grid4, block, 0, st, b, vwords, v, (const int*)cont,              \
+                       (const int*)count);                                                                   \
+    hipLaunchKernelGGL(k_vanta_pred_full<M>, grid, block, 0, st, b, vwords, v, (const int*)fresh, (const int*)count)
         if (maxc == 2) {
             RSX_LAUNCH_PRED(2);
         } else if (maxc == 4) {
