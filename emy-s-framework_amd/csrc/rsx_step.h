@@ -667,11 +667,19 @@ RSX_HD bool sphere_box(V3 c, float r, const Xf& box, V3 h, float margin, float b
     return true;
 }
 
+// Whether the lane steps its arena.  With pooled queries a lane that stores
+// nothing (it mirrors another lane's arena, or pads the grid past the
+// bucket's end) takes part only in the wavefront's collective steps, with no
+// work of its own, so that the pools spread the other lanes' work over it.
+RSX_HD bool tick_live(const TickCtx& T) { return T.own || !T.pool; }
+
 template <int MAXC>
 RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, const bool* far) {
     const float dt = T.dt;
     const int nb = 1 + ncars_of(A);
     const CarGeom& g = T.cg;
+    const bool live = tick_live(T);  // not live: the pools' call sites only
+    if (live) {
     // gravity on active bodies (applyGravity): m_gravity = acc * (1 / invMass)
     const V3 gacc = V3{0, 0, kGravityZ} * kUU2BT;
     if (!A.ball.sleeping) A.ball.force += gacc * (1.0f / T.bg.inv_mass);
@@ -707,6 +715,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         A.ball.bp_reg = bp_cell_index(T.mv, W.bmin[0]);
         for (int i = 0; i < ncars_of(A); i++) A.car[i].bp_reg = bp_cell_index(T.mv, W.bmin[1 + i]);
     }
+    }  // live
 
     // ---- pairs, manifolds, contacts (dispatch order) ----
     MfList<MAXC>& L = W.L;
@@ -723,8 +732,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         // response disabled; one demolished during this tick's collision
         // callbacks keeps both until the next tick (Car::_PreTickUpdate)
         bool demoed = (d >= 1) && !A.car[d - 1].active;
-        const int cell = bp_cell_index(T.mv, W.bmin[d]);
-        uint32_t mask = T.mv.bp_meshes[cell];
+        uint32_t mask = live ? T.mv.bp_meshes[bp_cell_index(T.mv, W.bmin[d])] : 0u;
         // statics: meshes then planes.  Every mesh pair that passes the AABB
         // tests opens its manifold, in this order, even when no triangle can
         // reach the body: Bullet hands every pair's manifold, empty or not, to
@@ -746,6 +754,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             mask = want ? (mask & cmask) : 0u;
         }
 #endif
+        if (!live) continue;
         if (!pooled_body && want) {
             if (d == 0) {
                 V3 r = V3{T.bg.radius + 0.08f, T.bg.radius + 0.08f, T.bg.radius + 0.08f};
@@ -915,6 +924,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         }
     }
 
+    if (!live) return;
     // ---- islands and manifold order (btSimulationIslandManager) ----
     int tag[MAXC + 1];
     for (int i = 0; i < nb; i++) tag[i] = uf_find(uf, i);
@@ -1176,19 +1186,22 @@ __device__ void cull_flush(const TickCtx& T, const TickScratch<MAXC>& W) {
 #endif
 
 // One full tick (Arena::Step body).  Returns true if the ball is in a goal
-// after the tick (Arena::IsBallScored).  With the learned cull enabled every
-// lane of the wavefront must call it (the cull's network is evaluated by the
-// whole wavefront).
+// after the tick (Arena::IsBallScored).  With the learned cull enabled or
+// pooled queries every lane of the wavefront must call it, and the
+// collective steps (the cull's network, the pools) are reached by all its
+// lanes at the same point: the work of a lane that is not live is skipped
+// around them.
 template <int MAXC>
 RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     const float dt = T.dt;
+    const bool live = tick_live(T);
     bool far[MAXC + 1];
     for (int b = 0; b <= MAXC; b++) far[b] = false;
 #if defined(__HIP_DEVICE_COMPILE__)
     if (T.cull.enabled) cull_far<MAXC>(A, T, W, far);
 #endif
     // ball zero-velocity sleep
-    A.ball.sleeping = (len2(A.ball.vel) == 0 && len2(A.ball.ang) == 0);
+    if (live) A.ball.sleeping = (len2(A.ball.vel) == 0 && len2(A.ball.ang) == 0);
     // Cars are visited last-added first, as Arena::Step's loops over
     // Arena::_cars (a std::unordered_set<Car*>) visit them: libstdc++ links a
     // node inserted into an empty bucket at the head of the set's list, so
@@ -1199,29 +1212,32 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     // of two cars on one pad locks it.
     for (int i = ncars_of(A) - 1; i >= 0; i--) {
         Car& c = A.car[i];
-        Controls& k = c.controls;
-        k.throttle = clampf(k.throttle, -1, 1);
-        k.steer = clampf(k.steer, -1, 1);
-        k.pitch = clampf(k.pitch, -1, 1);
-        k.yaw = clampf(k.yaw, -1, 1);
-        k.roll = clampf(k.roll, -1, 1);
-        c.active = true;
-        if (c.is_demoed) {
-            c.demo_respawn_timer = maxf(c.demo_respawn_timer - dt, 0);
-            if (c.demo_respawn_timer == 0) {
-                car_respawn(A, T, i);
-                far[1 + i] = false;  // moved after the cull was evaluated
-                W.cull_pred[1 + i] = -1e30f;
+        if (live) {
+            Controls& k = c.controls;
+            k.throttle = clampf(k.throttle, -1, 1);
+            k.steer = clampf(k.steer, -1, 1);
+            k.pitch = clampf(k.pitch, -1, 1);
+            k.yaw = clampf(k.yaw, -1, 1);
+            k.roll = clampf(k.roll, -1, 1);
+            c.active = true;
+            if (c.is_demoed) {
+                c.demo_respawn_timer = maxf(c.demo_respawn_timer - dt, 0);
+                if (c.demo_respawn_timer == 0) {
+                    car_respawn(A, T, i);
+                    far[1 + i] = false;  // moved after the cull was evaluated
+                    W.cull_pred[1 + i] = -1e30f;
+                }
+                c.active = false;  // body disabled for this tick
             }
-            c.active = false;  // body disabled for this tick
         }
 #if defined(__HIP_DEVICE_COMPILE__)
         if (T.pool) {
             // the mesh part of this car slot's wheel rays on every lane,
-            // spread over the wavefront (a demolished car casts none)
+            // spread over the wavefront (a demolished car casts none); the
+            // one call site every lane reaches together
             RayHit cast[4];
-            car_mesh_rays_pool(A, T, i, !c.is_demoed && !far[1 + i], cast);
-            if (c.is_demoed) continue;
+            car_mesh_rays_pool(A, T, i, live && !c.is_demoed && !far[1 + i], cast);
+            if (!live || c.is_demoed) continue;
             car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i], cast);
             continue;
         }
@@ -1229,7 +1245,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
         if (c.is_demoed) continue;
         car_pre_tick(A, i, T.cg, T.bg, T.mv, dt, dt, *T.sx, !far[1 + i]);
     }
-    if (ncars_of(A) > 0) {
+    if (live && ncars_of(A) > 0) {
         for (int p = 0; p < kNumPads; p++) {
             Pad& pad = A.pad[p];
             if (pad.cooldown > 0) pad.cooldown = maxf(pad.cooldown - dt, 0);
@@ -1240,6 +1256,7 @@ RSX_HD bool arena_tick(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W) {
     M3 rot0[MAXC > 0 ? MAXC : 1];
     for (int i = 0; i < ncars_of(A); i++) rot0[i] = A.car[i].xf.b;
     arena_world_step(A, T, W, far);
+    if (!live) return false;
     for (int i = ncars_of(A) - 1; i >= 0; i--) {  // post-tick and pads, last-added car first
         Car& c = A.car[i];
         // A car demolished during this step skips Car::_PostTickUpdate, which
