@@ -191,21 +191,60 @@ RSX_HD void contact_added(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int idx
     (void)T;
 }
 
+// Result of one pooled contact query (a car's gjk::closest_points, the
+// ball's sphere-triangle test), with the internal-edge correction of the
+// contact it adds (adjust_internal_edge).
+struct PooledContact {
+    V3 normal_on_b, point_on_b;
+    float depth;
+    int hit;          // bit 0: a point within reach; bit 1: its normal is corrected to `normal_fix`
+    V3 normal_fix;
+};
+
+// LDS of a wavefront's pooled contact queries: the 64 lanes' body
+// transforms, item counts, the item list (owner lane, triangle) and results.
+// Queries past kPoolCap run in their owner lane.
+constexpr int kPoolCap = 448;
+constexpr int kPoolXfOff = 0;
+constexpr int kPoolCntOff = kPoolXfOff + 64 * (int)sizeof(Xf);
+constexpr int kPoolItemOff = kPoolCntOff + 64 * 4;
+constexpr int kPoolResOff = kPoolItemOff + kPoolCap * 4;
+constexpr int kPoolBytes = kPoolResOff + kPoolCap * (int)sizeof(PooledContact);
+
 // Narrowphase of the ball against mesh `mesh` (convex-concave, swapped).
 // The candidates are sorted by mesh and the meshes visited in increasing
 // order, so the scan starts at `pos` (where the previous mesh's run ended)
-// and leaves it at the end of this mesh's run.
+// and leaves it at the end of this mesh's run.  With pooling the candidates
+// are the triangles the ball's sphere test kept, whose contacts (and their
+// internal-edge corrections) ran in body_mesh_pool: only their results are
+// read here, in the same order.
 template <int MAXC>
-RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, Manifold& m, int mesh, const int* cand, int ncand,
-                         int& pos) {
+RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, const TickScratch<MAXC>& W, Manifold& m, int mesh,
+                         int ncand, int& pos) {
     const Ball& b = A.ball;
+    const int* cand = W.cand;
     Xf trB = xf_identity();
+    const PooledContact* pooled =
+        T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
     int i = pos;
     for (; i < ncand; i++) {
         const int pk = cand[i];
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
+        if (pooled && W.pool_base + i < kPoolCap) {
+            const PooledContact& r = pooled[W.pool_base + i];
+            const int hit = r.hit;
+            if (hit) {
+                int idx = manifold_add(m, b.xf, trB, r.normal_on_b, r.point_on_b, r.depth, minf(kBallFriction, 0.6f),
+                                       maxf(kBallRestitution, 0.3f));
+                if (idx >= 0) {
+                    contact_added(A, T, m, idx);  // the edge correction ran with the query
+                    if (hit & 2) contact_renormal(m.pt[idx], r.normal_fix);
+                }
+            }
+            continue;
+        }
         V3 v0, v1, v2;
         load_tri(T.mv, t, v0, v1, v2);
         if (sphere_tri_early_out(b.xf.o, T.bg.radius, v0, v1, v2, m.breaking)) continue;
@@ -235,24 +274,6 @@ RSX_HD bool box_tri_separated(const Xf& child, const M3& inv_basis, V3 hm, const
     return false;
 }
 
-// Result of one pooled box-triangle query (gjk::closest_points), with the
-// internal-edge correction of the contact it adds (adjust_internal_edge).
-struct PooledContact {
-    V3 normal_on_b, point_on_b;
-    float depth;
-    int hit;          // bit 0: a point within reach; bit 1: its normal is corrected to `normal_fix`
-    V3 normal_fix;
-};
-
-// LDS of a wavefront's pooled box-triangle queries: the 64 lanes' car child
-// transforms, item counts, the item list (owner lane, triangle) and results.
-// Queries past kPoolCap run in their owner lane.
-constexpr int kPoolCap = 448;
-constexpr int kPoolXfOff = 0;
-constexpr int kPoolCntOff = kPoolXfOff + 64 * (int)sizeof(Xf);
-constexpr int kPoolItemOff = kPoolCntOff + 64 * 4;
-constexpr int kPoolResOff = kPoolItemOff + kPoolCap * 4;
-constexpr int kPoolBytes = kPoolResOff + kPoolCap * (int)sizeof(PooledContact);
 
 // LDS of the gather stage of a wavefront's pooled car queries, laid over the
 // results (written only after it): per lane its block's first entry, entry
@@ -275,21 +296,21 @@ static_assert(kGatKeptOff % 8 == 0 && kGatEnd <= kPoolBytes && kMaxCand <= 256, 
 // wavefront, spread over its 64 lanes: the candidate gather (gather_tris's
 // block scan under the body's box) and the filter (meshes outside the
 // broadphase mask `bp`; for a car box_tri_separated, for the ball the
-// sphere-triangle early-out and test of ball_vs_mesh), then for a car the
-// box-triangle queries (gjk::closest_points, then the contact's internal-edge
-// correction).  Each lane puts its block range, box and transform in LDS;
-// the wavefront tests the block entries 64 at a time, marking in the owner's
-// masks the meshes of its candidates and the entries the filter keeps; each
-// owner lists its kept triangles in key order (W.cand, `ncand` of them) --
-// for a car also as query items, which every lane then runs lane, lane + 64,
-// ... with its own GJK working set, the results waiting in LDS for
-// car_vs_mesh.  The ball's kept triangles are exactly those for which
-// ball_vs_mesh adds a point, which it then does over this shorter list.  One
-// lane's gather and tests would otherwise run one after another while the
-// rest of its wavefront idles.  Returns the meshes with candidates.  A lane
-// whose box spans more than two grid cells along some axis gathers with
-// gather_tris and filters in its narrowphase (W.pool_local).  `want` false:
-// no candidates.  Every lane of the wavefront must call it, with the same d.
+// sphere-triangle early-out and test of ball_vs_mesh), then the contact
+// queries (a car's gjk::closest_points, the ball's sphere-triangle contact,
+// each with the contact's internal-edge correction).  Each lane puts its
+// block range, box and transform in LDS; the wavefront tests the block
+// entries 64 at a time, marking in the owner's masks the meshes of its
+// candidates and the entries the filter keeps; each owner lists its kept
+// triangles in key order (W.cand, `ncand` of them) and as query items, which
+// every lane then runs lane, lane + 64, ..., the results waiting in LDS for
+// car_vs_mesh / ball_vs_mesh.  One lane's gather and tests would otherwise
+// run one after another while the rest of its wavefront idles.  Returns the
+// meshes with candidates.  A lane whose box spans more than two grid cells
+// along some axis gathers with gather_tris and runs its tests in its
+// narrowphase (W.pool_local); so does the ball of an arena without car
+// slots, over the kept list.  `want` false: no candidates.  Every lane of the
+// wavefront must call it, with the same d.
 template <int MAXC>
 __device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, int d, bool want,
                                    uint32_t bp, int& ncand) {
@@ -396,8 +417,12 @@ __device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickS
             }
         }
     }
-    if (d == 0) return cmask;  // the ball: no queries to pool
-  if constexpr (MAXC > 0) {
+    if constexpr (MAXC == 0) {
+        // an arena without car slots (its kernels hold no results array):
+        // ball_vs_mesh runs the kept triangles' tests in the lane
+        W.pool_local = true;
+        return cmask;
+    } else {
     const int kept = W.pool_local ? 0 : ncand;
     int kincl = kept;
     for (int o = 1; o < 64; o <<= 1) {
@@ -414,7 +439,32 @@ __device__ uint32_t body_mesh_pool(const Arena<MAXC>& A, const TickCtx& T, TickS
     const float max_d = g.box_margin + g.breaking;
     for (int it0 = 0; it0 < nq; it0 += 64) {
         const int it = it0 + l;
-        if (it < nq) {
+        if (it < nq && d == 0) {
+            // ball_vs_mesh's sphere-triangle contact, then contact_added's
+            // internal-edge correction
+            const uint32_t e = items[it];
+            V3 v0, v1, v2, pt, nrm;
+            float depth;
+            load_tri(T.mv, (int)(e & 0xffffu), v0, v1, v2);
+            PooledContact r;
+            r.hit = sphere_triangle(xfs[e >> 16].o, T.bg.radius, T.bg.breaking, v0, v1, v2, pt, nrm, depth) ? 1 : 0;
+            r.normal_on_b = nrm;
+            r.point_on_b = pt;
+            r.depth = depth;
+            if (r.hit && !(depth > T.bg.breaking)) {
+                Contact cp;
+                cp.posB = pt;
+                cp.posA = pt + nrm * depth;
+                cp.localB = inv_xform(xf_identity(), pt);
+                cp.normal = nrm;
+                cp.dist = depth;
+                if (adjust_internal_edge(T.mv, (int)(e & 0xffffu), cp)) {
+                    r.hit |= 2;
+                    r.normal_fix = cp.normal;
+                }
+            }
+            res[it] = r;
+        } else if (it < nq) {
             const uint32_t e = items[it];
             gjk::Shapes sh;
             load_tri(T.mv, (int)(e & 0xffffu), sh.tri[0], sh.tri[1], sh.tri[2]);
@@ -794,7 +844,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
                 if ((mask >> ms) & 1) {
-                    if (d == 0) ball_vs_mesh(A, T, m, ms, W.cand, ncand, cpos);
+                    if (d == 0) ball_vs_mesh(A, T, W, m, ms, ncand, cpos);
                     else car_vs_mesh(A, T, W, m, d - 1, ms, ncand, cpos);
                 }
             }
