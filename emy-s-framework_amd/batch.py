@@ -400,9 +400,9 @@ class RocketSimBatch:
     def Autotune(self, device_actions_ptr, env_steps=3):
         """Times every launch shape on the current arenas, keeps the fastest and
         restores the arenas; returns [(block_lanes, arenas_per_wave, pool_narrowphase, ms or None)]."""
-        res = (TuneResult * 32)()
+        res = (TuneResult * 44)()
         n = _I()
-        _check(self._lib.rsx_autotune(self._h, device_actions_ptr, env_steps, res, 32, C.byref(n)))
+        _check(self._lib.rsx_autotune(self._h, device_actions_ptr, env_steps, res, 44, C.byref(n)))
         return [(r.tuning.block_lanes, r.tuning.arenas_per_wave, r.tuning.pool_narrowphase,
                  r.ms_per_env_step if r.feasible else None) for r in res[:n.value]]
 

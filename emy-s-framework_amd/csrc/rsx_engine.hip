@@ -1276,13 +1276,22 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
     if (!e) return fail(RSX_NOT_INITIALIZED, "engine is null");
     if (!device_actions || env_steps < 1) return fail(RSX_INVALID_ARG, "need device actions and env_steps >= 1");
     HIP_TRY(hipSetDevice(e->device));
+    // Fewer than 8 arenas per wavefront only with pooled queries: the spare
+    // lanes then take their share of the pools; unpooled they would only
+    // replay arenas.
     static const int kBlocks[] = {64, 128, 256, 512};
-    static const int kApw[] = {64, 32, 16, 8};
+    static const int kApw[] = {64, 32, 16, 8, 4, 2, 1};
     std::vector<rsx_tune_result> res;
     for (int pool = 1; pool >= 0; pool--)
         for (int bl : kBlocks)
-            for (int apw : kApw) res.push_back(rsx_tune_result{{bl, apw, pool}, -1.f, 0});
-    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 32");
+            for (int apw : kApw)
+                if (pool || apw >= 8) res.push_back(rsx_tune_result{{bl, apw, pool}, -1.f, 0});
+    if (results && capacity < (int32_t)res.size()) return fail(RSX_INVALID_ARG, "results capacity below 44");
+    // a shape needing more than 8 wavefronts per SIMD for these arenas is
+    // reported infeasible without a trial
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+    const int64_t max_waves = 8LL * 4 * std::max(cus, 1);
     // Snapshot of everything a trial overwrites, restored after each trial
     // and at the end.
     std::vector<std::pair<void*, size_t>> bufs;
@@ -1325,6 +1334,9 @@ rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t en
         for (size_t k = 0; k < res.size(); k++) {
             const LaunchShape s{res[k].tuning.block_lanes, res[k].tuning.arenas_per_wave,
                                 res[k].tuning.pool_narrowphase};
+            int64_t waves = 0;
+            for (const auto& b : e->bucket) waves += arena_waves(b.n, s);
+            if (waves > max_waves) continue;  // more than 8 wavefronts per SIMD: not timed
             const size_t lds = arena_lds_per_wave(e->cull.enabled != 0, s.pool != 0) * (s.block / 64);
             if (lds > e->lds_per_block_max || prepare_arena_kernels(lds) != hipSuccess) {
                 (void)hipGetLastError();

@@ -61,7 +61,7 @@ int arena_waves(int n, const LaunchShape& s) { return (n + s.arenas_per_wave - 1
 bool shape_valid(const LaunchShape& s) {
     const bool block_ok = s.block == 64 || s.block == 128 || s.block == 256 || s.block == 512;
     const int a = s.arenas_per_wave;
-    return block_ok && (a == 64 || a == 32 || a == 16 || a == 8) && (s.pool == 0 || s.pool == 1);
+    return block_ok && a >= 1 && a <= 64 && (a & (a - 1)) == 0 && (s.pool == 0 || s.pool == 1);
 }
 
 }  // namespace rsx

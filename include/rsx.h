@@ -294,13 +294,17 @@ rsx_status rsx_synchronize(rsx_engine* e);
  *                     wavefronts per SIMD with at most 256 registers a lane;
  *                     its LDS fits only without pooling; the tick path runs
  *                     it as 256-lane workgroups);
- *   arenas_per_wave   arenas per 64-lane wavefront: 64, 32, 16 or 8 (below 64
- *                     the spare lanes repeat a lane's arena and store nothing,
- *                     which trades lanes for fewer diverging branches per
- *                     wavefront and more wavefronts in flight);
- *   pool_narrowphase  1: the box-triangle queries of all lanes of a
- *                     wavefront are spread over its 64 lanes; 0: each lane
- *                     runs its own.
+ *   arenas_per_wave   arenas per 64-lane wavefront: 64, 32, 16, 8, 4, 2 or 1
+ *                     (below 64 the spare lanes store nothing: with pooled
+ *                     queries they skip the tick's own work and take their
+ *                     share of the wavefront's pooled work, so that each
+ *                     arena's gathers, mesh queries and wheel-ray mesh tests
+ *                     are spread over 64 / arenas_per_wave lanes; unpooled
+ *                     they repeat a lane's arena);
+ *   pool_narrowphase  1: every body's mesh gather and contact queries and
+ *                     the wheel rays' mesh tests of all lanes of a wavefront
+ *                     are spread over its 64 lanes; 0: each lane runs its
+ *                     own.
  * Results do not depend on the shape.  Default {64, 64, 1}. */
 typedef struct rsx_tuning {
     int32_t block_lanes;
@@ -318,12 +322,13 @@ rsx_status rsx_set_tuning(rsx_engine* e, const rsx_tuning* t);
 rsx_status rsx_get_tuning(const rsx_engine* e, rsx_tuning* t);
 
 /* Autotuner: times `env_steps` env steps (after one untimed) of every shape
- * in {64, 128, 256, 512} x {64, 32, 16, 8} x {pooled, unpooled} on the
- * engine's current arenas with the given device actions (shapes whose LDS
- * does not fit a workgroup are reported infeasible), keeps the fastest, and
- * restores the arenas, the env outputs and the cull's counters to what they
- * were.  results (nullable, capacity >= 32) receives every shape's time;
- * *count the number written.
+ * in {64, 128, 256, 512} x ({64, 32, 16, 8, 4, 2, 1} pooled, {64, 32, 16, 8}
+ * unpooled) -- 44 shapes -- on the engine's current arenas with the given
+ * device actions (shapes whose LDS does not fit a workgroup, or that need
+ * more than 8 wavefronts per SIMD for these arenas, are reported infeasible
+ * without a trial), keeps the fastest, and restores the arenas, the env
+ * outputs and the cull's counters to what they were.  results (nullable,
+ * capacity >= 44) receives every shape's time; *count the number written.
  * Blocks the host until done. */
 rsx_status rsx_autotune(rsx_engine* e, const int32_t* device_actions, int32_t env_steps, rsx_tune_result* results,
                         int32_t capacity, int32_t* count);
