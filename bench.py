@@ -379,8 +379,13 @@ def main():
         if not args.autotune:
             return None
         apply_states(batch, seed=4)
+        # past the first env steps (freshly placed overlapping cars settle
+        # there, at twice the steady step time), as the timed steps will be
+        for i in range(args.warmup):
+            batch.StepEnv(actions[i % len(actions)].data_ptr(), sp)
+        torch.cuda.synchronize()
         d_act.copy_(actions[0])
-        res = batch.Autotune(d_act.data_ptr(), env_steps=2)
+        res = batch.Autotune(d_act.data_ptr(), env_steps=3)
         return [{"block": b, "arenas_per_wave": a, "pool": p, "ms": ms} for b, a, p, ms in res]
 
     def run(tag, steps, warmup, retune=True):
