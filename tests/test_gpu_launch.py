@@ -153,7 +153,8 @@ def test_mixed_launch_shapes_are_bit_identical(torch_cuda):
     acts = _actions(torch, b.num_cars, 3, 2)
     want = _run(b, acts)
     b.Destroy()
-    for shape in ((256, 16, 1), (128, 8, 0), (64, 64, 0)):
+    # with 2 and 1 arenas per wavefront most lanes only take part in the pools
+    for shape in ((256, 16, 1), (128, 8, 0), (64, 64, 0), (64, 2, 1), (128, 1, 1)):
         b = make()
         got = _run(b, acts, shape)
         b.Destroy()
