@@ -379,26 +379,51 @@ RSX_HD void solver_iterate_lds(Solver<MAXC>& S, float* col) {
         H.f(b, kHotInvMass) = s.inv_mass;
         H.f(b, kHotDyn) = s.dyn ? 1.f : 0.f;
     }
+    // Each sweep works on a register copy of its row, the next row's copy
+    // loaded before this one is solved (rows change only their own
+    // impulses, written back after the solve), so the rows' private-memory
+    // loads overlap the previous row's solve.
+    const int nr = S.nrow, nf = S.nfric;
     for (int it = 0; it < kSolverIterations; it++) {
         float res = 0.f;
-        for (int j = 0; j < S.nrow; j++) {
-            float r = hot_solve_split(H, S.row[j]);
-            res = maxf(res, r * r);
+        if (nr > 0) {
+            Row cur = S.row[0];
+            for (int j = 0; j < nr; j++) {
+                const Row nxt = S.row[j + 1 < nr ? j + 1 : j];
+                float r = hot_solve_split(H, cur);
+                S.row[j].push = cur.push;
+                res = maxf(res, r * r);
+                cur = nxt;
+            }
         }
         if (res <= 0.f || it >= (kSolverIterations - 1)) break;
     }
     for (int it = 0; it < kSolverIterations; it++) {
-        for (int j = 0; j < S.nrow; j++) {
-            if (S.row[j].special) continue;
-            hot_solve_row(H, S.row[j], false);
+        if (nr > 0) {
+            Row cur = S.row[0];
+            for (int j = 0; j < nr; j++) {
+                const Row nxt = S.row[j + 1 < nr ? j + 1 : j];
+                if (!cur.special) {
+                    hot_solve_row(H, cur, false);
+                    S.row[j].applied = cur.applied;
+                }
+                cur = nxt;
+            }
         }
-        for (int j = 0; j < S.nfric; j++) {
-            Row& f = S.fric[j];
-            float total = S.row[f.fidx].applied;
-            if (total > 0.f) {
-                f.lo = -(f.friction * total);
-                f.hi = f.friction * total;
-                hot_solve_row(H, f, true);
+        if (nf > 0) {
+            Row cur = S.fric[0];
+            for (int j = 0; j < nf; j++) {
+                const Row nxt = S.fric[j + 1 < nf ? j + 1 : j];
+                const float total = S.row[cur.fidx].applied;
+                if (total > 0.f) {
+                    cur.lo = -(cur.friction * total);
+                    cur.hi = cur.friction * total;
+                    hot_solve_row(H, cur, true);
+                    S.fric[j].lo = cur.lo;
+                    S.fric[j].hi = cur.hi;
+                    S.fric[j].applied = cur.applied;
+                }
+                cur = nxt;
             }
         }
     }
