@@ -263,6 +263,22 @@ RSX_HD void solve_row(Solver<MAXC>& S, Row& c, bool upper) {
     sbody_apply(B, c.n2 * B.inv_mass, c.angB, delta);
 }
 
+// One Gauss-Seidel sweep over rows[0, n): solve(home, c) solves register
+// copy c of row `home` and writes back the impulses it changed.  The next
+// row's copy is loaded before the current one is solved (a row changes only
+// its own impulses), so the rows' private-memory loads overlap the previous
+// row's solve.
+template <typename Solve>
+RSX_HD void sweep_rows(Row* rows, int n, Solve&& solve) {
+    if (n <= 0) return;
+    Row cur = rows[0];
+    for (int j = 0; j < n; j++) {
+        const Row nxt = rows[j + 1 < n ? j + 1 : j];
+        solve(rows[j], cur);
+        cur = nxt;
+    }
+}
+
 template <int MAXC>
 RSX_HD void solver_iterate(Solver<MAXC>& S) {
     for (int it = 0; it < kSolverIterations; it++) {
@@ -379,53 +395,33 @@ RSX_HD void solver_iterate_lds(Solver<MAXC>& S, float* col) {
         H.f(b, kHotInvMass) = s.inv_mass;
         H.f(b, kHotDyn) = s.dyn ? 1.f : 0.f;
     }
-    // Each sweep works on a register copy of its row, the next row's copy
-    // loaded before this one is solved (rows change only their own
-    // impulses, written back after the solve), so the rows' private-memory
-    // loads overlap the previous row's solve.
     const int nr = S.nrow, nf = S.nfric;
     for (int it = 0; it < kSolverIterations; it++) {
         float res = 0.f;
-        if (nr > 0) {
-            Row cur = S.row[0];
-            for (int j = 0; j < nr; j++) {
-                const Row nxt = S.row[j + 1 < nr ? j + 1 : j];
-                float r = hot_solve_split(H, cur);
-                S.row[j].push = cur.push;
-                res = maxf(res, r * r);
-                cur = nxt;
-            }
-        }
+        sweep_rows(S.row, nr, [&](Row& home, Row& c) {
+            const float r = hot_solve_split(H, c);
+            home.push = c.push;
+            res = maxf(res, r * r);
+        });
         if (res <= 0.f || it >= (kSolverIterations - 1)) break;
     }
     for (int it = 0; it < kSolverIterations; it++) {
-        if (nr > 0) {
-            Row cur = S.row[0];
-            for (int j = 0; j < nr; j++) {
-                const Row nxt = S.row[j + 1 < nr ? j + 1 : j];
-                if (!cur.special) {
-                    hot_solve_row(H, cur, false);
-                    S.row[j].applied = cur.applied;
-                }
-                cur = nxt;
+        sweep_rows(S.row, nr, [&](Row& home, Row& c) {
+            if (c.special) return;
+            hot_solve_row(H, c, false);
+            home.applied = c.applied;
+        });
+        sweep_rows(S.fric, nf, [&](Row& home, Row& c) {
+            const float total = S.row[c.fidx].applied;
+            if (total > 0.f) {
+                c.lo = -(c.friction * total);
+                c.hi = c.friction * total;
+                hot_solve_row(H, c, true);
+                home.lo = c.lo;
+                home.hi = c.hi;
+                home.applied = c.applied;
             }
-        }
-        if (nf > 0) {
-            Row cur = S.fric[0];
-            for (int j = 0; j < nf; j++) {
-                const Row nxt = S.fric[j + 1 < nf ? j + 1 : j];
-                const float total = S.row[cur.fidx].applied;
-                if (total > 0.f) {
-                    cur.lo = -(cur.friction * total);
-                    cur.hi = cur.friction * total;
-                    hot_solve_row(H, cur, true);
-                    S.fric[j].lo = cur.lo;
-                    S.fric[j].hi = cur.hi;
-                    S.fric[j].applied = cur.applied;
-                }
-                cur = nxt;
-            }
-        }
+        });
     }
     for (int b = 0; b < MAXC + 2; b++) {
         SBody& s = S.body[b];
