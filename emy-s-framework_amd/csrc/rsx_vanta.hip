@@ -140,19 +140,33 @@ __global__ void __launch_bounds__(64) k_vanta_pred_cont(BucketArgs B, uint32_t* 
     }
 }
 
-// One wavefront per workgroup.  The ball's mesh gather and sphere-triangle
-// tests are pooled over the wavefront's lanes (body_mesh_pool, in the
-// workgroup's static LDS), so every lane runs all 360 ticks: lanes past the
-// list's end run a copy of its last arena and store nothing.
+// Arenas per wavefront of the new rollouts: enough wavefronts for about two
+// per SIMD of the device (the kernel's occupancy), at least kMinRolloutApw
+// arenas each.  A list of a few thousand arenas thus spreads each ball's
+// pooled mesh work over 16 lanes instead of leaving most SIMDs idle.
+constexpr int kMinRolloutApw = 4;
+__device__ inline int rollout_apw(int n) {
+    int k = kMinRolloutApw;
+    while (k < 64 && (n + k - 1) / k > 2048) k *= 2;
+    return k;
+}
+
+// One wavefront per workgroup, rollout_apw(n) listed arenas per wavefront.
+// The ball's mesh gather and sphere-triangle tests are pooled over the
+// wavefront's lanes (body_mesh_pool, in the workgroup's static LDS): the
+// lanes that own no listed arena (past the wavefront's share or the list's
+// end) take only their part of the pools and store nothing.
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_vanta_pred_full(BucketArgs B, uint32_t* vwords, VantaArgs V, const int* work,
                                                         const int* count) {
     __shared__ __align__(16) uint8_t pool[kGatEnd];
     const int n = count[0];
-    if ((int)blockIdx.x * 64 >= n) return;  // the whole wavefront
-    const int i = blockIdx.x * 64 + (int)threadIdx.x;
-    const bool own = i < n;
-    const int a = work[own ? i : n - 1];
+    const int k = rollout_apw(n);
+    if ((int)blockIdx.x * k >= n) return;  // the whole wavefront
+    const int l = (int)threadIdx.x;
+    const int i = blockIdx.x * k + (l & (k - 1));
+    const bool own = l < k && i < n;
+    const int a = work[i < n ? i : n - 1];
     const Col col{B.words + a, B.n};
     const Col vc{vwords + a, B.n};
     const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
@@ -206,6 +220,7 @@ hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const u
                         int* lists, hipStream_t st) {
     if (!b.n) return hipSuccess;
     dim3 grid((b.n + 63) / 64), grid4((4 * b.n + 63) / 64), block(64);
+    dim3 grid_full((b.n + kMinRolloutApw - 1) / kMinRolloutApw);  // rollout_apw(n) >= kMinRolloutApw
     int* const fresh = lists;
     int* const cont = lists + b.n;
     int* const count = lists + 2 * (size_t)b.n;
@@ -216,7 +231,8 @@ hipError_t launch_vanta(int maxc, const BucketArgs& b, uint32_t* vwords, const u
     hipLaunchKernelGGL(k_vanta_pred_step<M>, grid, block, 0, st, b, vwords, reset, v, fresh, cont, count);     \
     hipLaunchKernelGGL(k_vanta_pred_cont<M>, grid4, block, 0, st, b, vwords, v, (const int*)cont,              \
                        (const int*)count);                                                                   \
-    hipLaunchKernelGGL(k_vanta_pred_full<M>, grid, block, 0, st, b, vwords, v, (const int*)fresh, (const int*)count)
+    hipLaunchKernelGGL(k_vanta_pred_full<M>, grid_full, block, 0, st, b, vwords, v, (const int*)fresh,          \
+                       (const int*)count)
         if (maxc == 2) {
             RSX_LAUNCH_PRED(2);
         } else if (maxc == 4) {
