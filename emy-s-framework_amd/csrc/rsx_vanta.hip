@@ -63,26 +63,41 @@ __global__ void k_vanta_count_zero(int* count) {
     if (threadIdx.x < 2) count[threadIdx.x] = 0;
 }
 
+// Appends `value` of every lane with `take` to list (length *count), one
+// atomic per wavefront: the lanes' slots follow their order in the
+// wavefront.  Every lane of the wavefront must call it.
+__device__ inline void wave_append(int* list, int* count, bool take, int value) {
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
 template <int MAXC>
 __global__ void __launch_bounds__(64) k_vanta_pred_step(BucketArgs B, uint32_t* vwords, const uint8_t* reset,
                                                         VantaArgs V, int* fresh, int* cont, int* count) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= B.n) return;
-    const Col col{B.words + a, B.n};
-    const Col vc{vwords + a, B.n};
-    if (reset && reset[B.global[a]]) vanta::clear<MAXC>(vc);
-    const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
-    const uint64_t tick = col.get64(Layout<MAXC>::kArenaBase + kArTickLo);
-    const bool chains = (vc.get(vanta::kVaAvailHi) & vanta::kVaChainsBit) != 0;
-    const uint64_t t0 = vc.get64(vanta::kVaT0);
-    const int ts = V.Q.ticks_between;
-    if (chains && tick == t0 && vanta::same_ball(now, vanta::get_q(vc, vanta::kVaStart))) return;  // nothing moved
-    if (chains && vanta::chains_fit(ts) && tick == t0 + (uint64_t)ts &&
-        vanta::same_ball(now, vanta::get_q(vc, vanta::kVaChain))) {
-        cont[atomicAdd(&count[1], 1)] = a;
-        return;
+    bool go_cont = false, go_fresh = false;
+    if (a < B.n) {
+        const Col col{B.words + a, B.n};
+        const Col vc{vwords + a, B.n};
+        if (reset && reset[B.global[a]]) vanta::clear<MAXC>(vc);
+        const vanta::BallQ now = arena_ball_q(col, Layout<MAXC>::kBallBase);
+        const uint64_t tick = col.get64(Layout<MAXC>::kArenaBase + kArTickLo);
+        const bool chains = (vc.get(vanta::kVaAvailHi) & vanta::kVaChainsBit) != 0;
+        const uint64_t t0 = vc.get64(vanta::kVaT0);
+        const int ts = V.Q.ticks_between;
+        const bool moved = !(chains && tick == t0 && vanta::same_ball(now, vanta::get_q(vc, vanta::kVaStart)));
+        go_cont = moved && chains && vanta::chains_fit(ts) && tick == t0 + (uint64_t)ts &&
+                  vanta::same_ball(now, vanta::get_q(vc, vanta::kVaChain));
+        go_fresh = moved && !go_cont;
     }
-    fresh[atomicAdd(&count[0], 1)] = a;
+    wave_append(cont, &count[1], go_cont, a);
+    wave_append(fresh, &count[0], go_fresh, a);
 }
 
 // A ball-only arena's tick context with the pooled ball gather in `pool`.
