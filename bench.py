@@ -243,8 +243,9 @@ def vanta_bench(batch, apply_states, stream, sp, actions, reps=10):
     timed with HIP events on the launch stream: a build from freshly set
     states (every arena a new 360-tick rollout), then env steps each followed
     by the build (arenas whose ball moved as predicted continue their last
-    rollout), against the same env steps with DefaultObs.  Restores
-    DefaultObs."""
+    rollout), against the same env steps (the first `reps` after the same
+    freshly set states) with VantaObs without prediction and with
+    DefaultObs.  Restores DefaultObs."""
     import torch
     eng = _import_package()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -267,6 +268,7 @@ def vanta_bench(batch, apply_states, stream, sp, actions, reps=10):
     full_ms = timed(lambda i: batch.BuildObs(sp), 1)
     step_ms = timed(lambda i: batch.StepEnv(actions[i % len(actions)].data_ptr(), sp), reps)
     batch.ConfigureVanta(predict_ball=False)
+    apply_states(batch, seed=4)  # the same env steps as the two runs above
     nopred_ms = timed(lambda i: batch.StepEnv(actions[i % len(actions)].data_ptr(), sp), reps)
     batch.ConfigureEnv(no_touch_timeout_s=10.0, auto_reset=True)
     print(f"# VantaObs: build with every rollout new {full_ms:.2f} ms; env step + build {step_ms:.2f} ms "
@@ -276,7 +278,7 @@ def vanta_bench(batch, apply_states, stream, sp, actions, reps=10):
             "env_steps_per_s": batch.num_arenas / (step_ms * 1e-3),
             "rollout": "360 ball-only ticks per arena (slices at 1, 2, 3 s); an arena whose ball is, bit for bit, the "
                        "one its last rollout passed at the env step's tick skip continues that rollout instead "
-                       f"({reps} env steps after freshly set states)"}
+                       f"({reps} env steps after freshly set states, each variant from the same states)"}
 
 
 def measured_traffic(workload):
