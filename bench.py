@@ -1,9 +1,10 @@
 """Env-step throughput of the gfx950 arena engine on one MI355X.
 
-Workload (default): 65536 soccar 1v1 arenas from the random-state workload
-(emy-s-framework_amd/workloads.py, seed 4), a fresh uniform discrete action
-per car each step (device int32 indices over the 90-entry default table),
-tick-skip 8 at 120 Hz, exact collision.  One step = every arena advances one
+Workload (default): 65536 soccar 1v1 arenas in the contact-heavy states
+(emy-s-framework_amd/workloads.py stress_states, seed 4; --workload random or
+mixed for the others), a fresh uniform discrete action per car each step
+(device int32 indices over the 90-entry default table), tick-skip 8 at
+120 Hz, timed with exact collision and with the learned cull.  One step = every arena advances one
 env step: 8 physics ticks, the four stub rewards, goal / no-touch terminals,
 kickoff auto-reset of terminal arenas and each car's DefaultObs row, all in
 one fused launch (rsx_step_env).  Inputs are resident in HBM when the timed
