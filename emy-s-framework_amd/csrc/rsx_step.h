@@ -639,11 +639,12 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
     Car& c = A.car[ci];
     const Xf child = car_child_xf(g, c.xf);
     const Xf trB = xf_identity();
-    const M3 inv_basis = inverse(child.b);
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
     const float max_d = g.box_margin + 0.f + m.breaking;
     const PooledContact* pooled =
         T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
+    M3 inv_basis{};  // for the triangle early-out, which only the lane path runs
+    if (!pooled) inv_basis = inverse(child.b);
     int i = pos;
     for (; i < ncand; i++) {
         const int pk = cand[i];
@@ -822,22 +823,22 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             mask = 0;
         }
         if (demoed) continue;  // no contact response: never solved
+        // the body's broadphase AABB and (a car) its compound child's AABB,
+        // which no contact of this body changes
+        const V3 bmn = W.bmin[d], bmx = W.bmax[d];
+        V3 cmn = V3{0, 0, 0}, cmx = V3{0, 0, 0};
+        if (d >= 1) box_child_aabb(g, car_child_xf(g, A.car[d - 1].xf), cmn, cmx);
         int cpos = 0;  // the candidates not yet consumed by an earlier mesh
         for (int ms = 0; ms < kNumMeshes; ms++) {
             if (!((open >> ms) & 1)) continue;
             const float* ma = T.mv.mesh_aabb + ms * 6;
             const float th = kContactBreakingThreshold;
             V3 smn = V3{ma[0], ma[1], ma[2]}, smx = V3{ma[3], ma[4], ma[5]};  // shape AABB
-            if (!aabb_overlap(W.bmin[d], W.bmax[d], smn - V3{th, th, th}, smx + V3{th, th, th})) continue;
+            if (!aabb_overlap(bmn, bmx, smn - V3{th, th, th}, smx + V3{th, th, th})) continue;
             if (!active[d]) continue;  // needsCollision: both inactive
             float breaking = (d == 0) ? T.bg.breaking : g.breaking;
-            if (d >= 1) {
-                // compound child AABB against the mesh's shape AABB (no threshold)
-                Xf child = car_child_xf(g, A.car[d - 1].xf);
-                V3 cmn, cmx;
-                box_child_aabb(g, child, cmn, cmx);
-                if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
-            }
+            // a car: compound child AABB against the mesh's shape AABB (no threshold)
+            if (d >= 1 && !aabb_overlap(cmn, cmx, smn, smx)) continue;
             int k = mf_open(L, d, -1, breaking);
             int s = k >= 0 ? L.slot[k] : -1;
             if (s >= 0) {
@@ -855,18 +856,13 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             plane_shape_aabb(p, pmn, pmx);
             {
                 const float th = kContactBreakingThreshold;
-                if (!aabb_overlap(W.bmin[d], W.bmax[d], pmn - V3{th, th, th}, pmx + V3{th, th, th})) continue;
+                if (!aabb_overlap(bmn, bmx, pmn - V3{th, th, th}, pmx + V3{th, th, th})) continue;
             }
             if (!active[d]) continue;
             float breaking = (d == 0) ? T.bg.breaking : g.breaking;
-            if (d >= 1) {
-                // compound child AABB test against the plane's own AABB (no threshold)
-                Xf child = car_child_xf(g, A.car[d - 1].xf);
-                V3 cmn, cmx;
-                box_child_aabb(g, child, cmn, cmx);
-                // a child whose AABB misses is dropped with its manifold
-                if (!aabb_overlap(cmn, cmx, pmn, pmx)) continue;
-            }
+            // a car: compound child AABB test against the plane's own AABB (no
+            // threshold); a child whose AABB misses is dropped with its manifold
+            if (d >= 1 && !aabb_overlap(cmn, cmx, pmn, pmx)) continue;
             int k = mf_open(L, d, -1, breaking);
             int s = k >= 0 ? L.slot[k] : -1;
             if (s >= 0) {
