@@ -80,10 +80,13 @@ RSX_HD float restitution_curve(float rel_vel, float restitution) {
 }
 
 template <int MAXC>
-RSX_HD void setup_contact_row(Solver<MAXC>& S, Row& r, int ia, int ib, V3 normal, float dist, float friction,
+RSX_HD void setup_contact_row(Solver<MAXC>& S, Row& out, int ia, int ib, V3 normal, float dist, float friction,
                               float restitution, bool special, V3 rel1, V3 rel2, float dt) {
-    SBody& A = S.body[ia];
-    SBody& B = S.body[ib];
+    // the bodies read before the row is written, the row built in registers
+    // and stored once (row and body stores cannot then reorder the reads)
+    const SBody A = S.body[ia];
+    const SBody B = S.body[ib];
+    Row r{};
     r.a = (int8_t)ia;
     r.b = (int8_t)ib;
     const float relaxation = 1.f;
@@ -138,14 +141,16 @@ RSX_HD void setup_contact_row(Solver<MAXC>& S, Row& r, int ia, int ib, V3 normal
         r.hi = 1e10f;
     }
     r.special = special;
+    out = r;
 }
 
 template <int MAXC>
 RSX_HD void add_friction_row(Solver<MAXC>& S, V3 dir, int ia, int ib, float friction, V3 rel1, V3 rel2, int fidx) {
     if (S.nfric >= kMaxRows) return;
-    Row& r = S.fric[S.nfric++];
-    SBody& A = S.body[ia];
-    SBody& B = S.body[ib];
+    Row& out = S.fric[S.nfric++];
+    const SBody A = S.body[ia];
+    const SBody B = S.body[ib];
+    Row r{};
     r.a = (int8_t)ia;
     r.b = (int8_t)ib;
     r.fidx = (int16_t)fidx;
@@ -179,6 +184,7 @@ RSX_HD void add_friction_row(Solver<MAXC>& S, V3 dir, int ia, int ib, float fric
     r.rhs_pen = 0.f;
     r.lo = -friction;
     r.hi = friction;
+    out = r;
 }
 
 // convertContactInner without rolling friction (all rolling frictions are 0)
