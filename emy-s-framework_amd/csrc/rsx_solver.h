@@ -145,9 +145,10 @@ RSX_HD void setup_contact_row(Solver<MAXC>& S, Row& out, int ia, int ib, V3 norm
 }
 
 template <int MAXC>
-RSX_HD void add_friction_row(Solver<MAXC>& S, V3 dir, int ia, int ib, float friction, V3 rel1, V3 rel2, int fidx) {
-    if (S.nfric >= kMaxRows) return;
-    Row& out = S.fric[S.nfric++];
+RSX_HD void add_friction_row(Solver<MAXC>& S, int& nfric, V3 dir, int ia, int ib, float friction, V3 rel1, V3 rel2,
+                             int fidx) {
+    if (nfric >= kMaxRows) return;
+    Row& out = S.fric[nfric++];
     const SBody A = S.body[ia];
     const SBody B = S.body[ib];
     Row r{};
@@ -189,8 +190,8 @@ RSX_HD void add_friction_row(Solver<MAXC>& S, V3 dir, int ia, int ib, float fric
 
 // convertContactInner without rolling friction (all rolling frictions are 0)
 template <int MAXC>
-RSX_HD void add_contact_friction(Solver<MAXC>& S, int ia, int ib, V3 normal, float friction, V3 rel1, V3 rel2,
-                                 int fidx) {
+RSX_HD void add_contact_friction(Solver<MAXC>& S, int& nfric, int ia, int ib, V3 normal, float friction, V3 rel1,
+                                 V3 rel2, int fidx) {
     SBody& A = S.body[ia];
     SBody& B = S.body[ib];
     V3 v1 = A.dyn ? (A.lin + A.ext_f + cross(A.ang + A.ext_t, rel1)) : V3{0, 0, 0};
@@ -201,11 +202,11 @@ RSX_HD void add_contact_friction(Solver<MAXC>& S, int ia, int ib, V3 normal, flo
     float l2 = len2(lat);
     if (l2 > kEps) {
         lat *= 1.f / sqrtf(l2);
-        add_friction_row(S, lat, ia, ib, friction, rel1, rel2, fidx);
+        add_friction_row(S, nfric, lat, ia, ib, friction, rel1, rel2, fidx);
     } else {
         V3 p, q;
         plane_space(normal, p, q);
-        add_friction_row(S, p, ia, ib, friction, rel1, rel2, fidx);
+        add_friction_row(S, nfric, p, ia, ib, friction, rel1, rel2, fidx);
     }
 }
 

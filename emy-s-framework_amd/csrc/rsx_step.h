@@ -1018,6 +1018,9 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         S.sp_friction[i] = 0;
         S.sp_restitution[i] = 0;
     }
+    // the row counts in registers, stored once (the rows' stores would
+    // otherwise hold back each reread)
+    int nrow = 0, nfric = 0;
     for (int oi = 0; oi < nm; oi++) {
         int k = order[oi];
         int s = k >= 0 ? L.slot[k] : -1;
@@ -1029,14 +1032,14 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
         Xf xa = body_xf(A, m.body0);
         Xf xb = body_xf(A, m.body1);
         for (int j = 0; j < m.n; j++) {
-            if (S.nrow >= kMaxRows) break;
+            if (nrow >= kMaxRows) break;
             const Contact& cp = m.pt[j];
             V3 rel1 = cp.posA - xa.o;
             V3 rel2 = cp.posB - xb.o;
-            int ridx = S.nrow++;
+            int ridx = nrow++;
             Row& r = S.row[ridx];
             setup_contact_row(S, r, ia, ib, cp.normal, cp.dist, cp.friction, cp.restitution, cp.special, rel1, rel2, dt);
-            r.fidx = (int16_t)S.nfric;
+            r.fidx = (int16_t)nfric;
             if (cp.special) {
                 // only non-static bodies accumulate (the world side is static here)
                 int bi = m.body0;
@@ -1046,22 +1049,25 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 S.sp_normal[bi] += cp.normal;
                 S.sp_dist[bi] += len(rel1);
             }
-            add_contact_friction(S, ia, ib, cp.normal, cp.friction, rel1, rel2, ridx);
+            add_contact_friction(S, nfric, ia, ib, cp.normal, cp.friction, rel1, rel2, ridx);
         }
     }
     // special rows, in body order (only the ball produces them)
     for (int bi = 0; bi < nb; bi++) {
-        if (S.nspecial[bi] <= 0 || S.nrow >= kMaxRows) continue;
+        if (S.nspecial[bi] <= 0 || nrow >= kMaxRows) continue;
         float distance = S.sp_dist[bi] / (float)S.nspecial[bi];
         V3 normal = S.sp_normal[bi] / (float)S.nspecial[bi];
-        int ridx = S.nrow++;
+        int ridx = nrow++;
         Row& r = S.row[ridx];
         V3 rel1 = normal * -distance;
         V3 rel2 = V3{0, 0, 0};
         setup_contact_row(S, r, bi, FIX, normal, distance, S.sp_friction[bi], S.sp_restitution[bi], false, rel1, rel2,
                           dt);
-        add_contact_friction(S, bi, FIX, normal, S.sp_friction[bi], rel1, rel2, ridx);
+        add_contact_friction(S, nfric, bi, FIX, normal, S.sp_friction[bi], rel1, rel2, ridx);
     }
+
+    S.nrow = nrow;
+    S.nfric = nfric;
 
     if (T.slv) solver_iterate_lds(S, T.slv);
     else solver_iterate(S);
