@@ -118,23 +118,54 @@ RSX_HD void manifold_remove(Manifold& m, int i) {
     m.n--;
 }
 
-// btPersistentManifold::refreshContactPoints
+// btPersistentManifold::refreshContactPoints.  The points' fields are read
+// once into registers (the per-point stores would otherwise hold back the
+// next point's reads) and the removal tests use the refreshed values there;
+// the stored points are updated and removed exactly as the tests decide.
 RSX_HD void manifold_refresh(Manifold& m, const Xf& trA, const Xf& trB) {
-    for (int i = m.n - 1; i >= 0; i--) {
-        Contact& c = m.pt[i];
-        c.posA = xform(trA, c.localA);
-        c.posB = xform(trB, c.localB);
-        c.dist = dot(c.posA - c.posB, c.normal);
+    const int n = m.n;
+    const float brk = m.breaking;
+    V3 pa[kMaxManifoldPoints], pb[kMaxManifoldPoints], nr[kMaxManifoldPoints];
+    float d[kMaxManifoldPoints];
+#pragma unroll
+    for (int i = 0; i < kMaxManifoldPoints; i++) {
+        if (i >= n) break;
+        const Contact& c = m.pt[i];
+        pa[i] = xform(trA, c.localA);
+        pb[i] = xform(trB, c.localB);
+        nr[i] = c.normal;
+        d[i] = dot(pa[i] - pb[i], nr[i]);
     }
-    for (int i = m.n - 1; i >= 0; i--) {
+#pragma unroll
+    for (int i = 0; i < kMaxManifoldPoints; i++) {
+        if (i >= n) break;
         Contact& c = m.pt[i];
-        if (!(c.dist <= m.breaking)) {
-            manifold_remove(m, i);
+        c.posA = pa[i];
+        c.posB = pb[i];
+        c.dist = d[i];
+    }
+#pragma unroll
+    for (int i = kMaxManifoldPoints - 1; i >= 0; i--) {
+        if (i >= m.n) continue;
+        bool drop;
+        if (!(d[i] <= brk)) {
+            drop = true;
         } else {
-            V3 proj = c.posA - c.normal * c.dist;
-            V3 diff = c.posB - proj;
+            V3 proj = pa[i] - nr[i] * d[i];
+            V3 diff = pb[i] - proj;
             float d2 = dot(diff, diff);
-            if (d2 > m.breaking * m.breaking) manifold_remove(m, i);
+            drop = d2 > brk * brk;
+        }
+        if (drop) {
+            // manifold_remove: the last point takes slot i (mirrored in registers)
+            const int last = m.n - 1;
+            manifold_remove(m, i);
+            if (i != last) {
+                pa[i] = pa[last];
+                pb[i] = pb[last];
+                nr[i] = nr[last];
+                d[i] = d[last];
+            }
         }
     }
 }

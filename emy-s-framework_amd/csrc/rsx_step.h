@@ -227,8 +227,10 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, const TickScratch<MAX
     const PooledContact* pooled =
         T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
     int i = pos;
+    int pk_next = i < ncand ? cand[i] : 0;
     for (; i < ncand; i++) {
-        const int pk = cand[i];
+        const int pk = pk_next;
+        if (i + 1 < ncand) pk_next = cand[i + 1];  // the next key's load overlaps this one's contact
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
@@ -646,8 +648,10 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
     M3 inv_basis{};  // for the triangle early-out, which only the lane path runs
     if (!pooled) inv_basis = inverse(child.b);
     int i = pos;
+    int pk_next = i < ncand ? cand[i] : 0;
     for (; i < ncand; i++) {
-        const int pk = cand[i];
+        const int pk = pk_next;
+        if (i + 1 < ncand) pk_next = cand[i + 1];  // the next key's load overlaps this one's contact
         if (cand_mesh(pk) < mesh) continue;
         if (cand_mesh(pk) > mesh) break;
         const int t = cand_tri(pk);
