@@ -192,26 +192,36 @@ struct MfList {
     Manifold live[kLive];
 };
 
+// The list's two counters while a tick's pairs are opened, kept in
+// registers (the pair records' stores would otherwise hold back each
+// reread); mf_finish stores them.
+struct MfCursor {
+    int count = 0, nlive = 0;
+};
+
 // Returns the pair's index, or -1 when the pair list is full (the pair is
-// then skipped for this tick and the overflow flag set).
+// then skipped for this tick and the overflow flag set); `slot` receives the
+// manifold storage it got (-1: none).
 template <int MAXC>
-RSX_HD int mf_open(MfList<MAXC>& L, int b0, int b1, float breaking) {
-    if (L.count >= MfList<MAXC>::kCap) {
+RSX_HD int mf_open(MfList<MAXC>& L, MfCursor& C, int b0, int b1, float breaking, int& slot) {
+    slot = -1;
+    if (C.count >= MfList<MAXC>::kCap) {
         L.overflow = true;
         return -1;
     }
-    int k = L.count++;
+    int k = C.count++;
     L.body0[k] = (int8_t)b0;
     L.body1[k] = (int8_t)b1;
     L.slot[k] = -1;
-    if (L.nlive < MfList<MAXC>::kLive) {
-        int s = L.nlive;
+    if (C.nlive < MfList<MAXC>::kLive) {
+        int s = C.nlive;
         Manifold& m = L.live[s];
         m.body0 = b0;
         m.body1 = b1;
         m.n = 0;
         m.breaking = breaking;
         L.slot[k] = (int8_t)s;
+        slot = s;
     } else {
         L.overflow = true;
     }
@@ -219,12 +229,16 @@ RSX_HD int mf_open(MfList<MAXC>& L, int b0, int b1, float breaking) {
 }
 // Keep the manifold's storage only if it ended the tick with contacts.
 template <int MAXC>
-RSX_HD void mf_close(MfList<MAXC>& L, int k) {
-    if (k < 0) return;
-    int s = L.slot[k];
-    if (s < 0) return;
-    if (L.live[s].n > 0 && s == L.nlive) L.nlive++;
-    else if (L.live[s].n == 0) L.slot[k] = -1;
+RSX_HD void mf_close(MfList<MAXC>& L, MfCursor& C, int k, int s) {
+    if (k < 0 || s < 0) return;
+    const int n = L.live[s].n;
+    if (n > 0 && s == C.nlive) C.nlive++;
+    else if (n == 0) L.slot[k] = -1;
+}
+template <int MAXC>
+RSX_HD void mf_finish(MfList<MAXC>& L, const MfCursor& C) {
+    L.count = C.count;
+    L.nlive = C.nlive;
 }
 
 // --- AABBs (btCollisionWorld::updateSingleAabb) --------------------------------

@@ -776,6 +776,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
     MfList<MAXC>& L = W.L;
     L.count = 0;
     L.nlive = 0;
+    MfCursor C;
     int uf[MAXC + 1];
     for (int i = 0; i < nb; i++) uf[i] = i;
     bool active[MAXC + 1];
@@ -843,8 +844,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             float breaking = (d == 0) ? T.bg.breaking : g.breaking;
             // a car: compound child AABB against the mesh's shape AABB (no threshold)
             if (d >= 1 && !aabb_overlap(cmn, cmx, smn, smx)) continue;
-            int k = mf_open(L, d, -1, breaking);
-            int s = k >= 0 ? L.slot[k] : -1;
+            int s;
+            int k = mf_open(L, C, d, -1, breaking, s);
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + ms;
@@ -853,7 +854,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                     else car_vs_mesh(A, T, W, m, d - 1, ms, ncand, cpos);
                 }
             }
-            mf_close(L, k);
+            mf_close(L, C, k, s);
         }
         for (int p = 0; p < kNumPlanes; p++) {
             V3 pmn, pmx;
@@ -867,8 +868,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
             // a car: compound child AABB test against the plane's own AABB (no
             // threshold); a child whose AABB misses is dropped with its manifold
             if (d >= 1 && !aabb_overlap(cmn, cmx, pmn, pmx)) continue;
-            int k = mf_open(L, d, -1, breaking);
-            int s = k >= 0 ? L.slot[k] : -1;
+            int s;
+            int k = mf_open(L, C, d, -1, breaking, s);
             if (s >= 0) {
                 Manifold& m = L.live[s];
                 m.body1 = kBodyStatic + kNumMeshes + p;
@@ -903,7 +904,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                     if (m.n) manifold_refresh(m, c.xf, trB);
                 }
             }
-            mf_close(L, k);
+            mf_close(L, C, k, s);
         }
         // dynamic pairs with later bodies
         for (int e = d + 1; e < nb; e++) {
@@ -925,8 +926,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 ball_shape_aabb(T.bg, A.ball.xf.o, smn, smx);
                 if (!aabb_overlap(cmn, cmx, smn, smx)) continue;
                 float breaking = minf(T.bg.breaking, g.breaking);
-                int k = mf_open(L, 0, e, breaking);
-                int s = k >= 0 ? L.slot[k] : -1;
+                int s;
+                int k = mf_open(L, C, 0, e, breaking, s);
                 if (s >= 0) {
                     Manifold& m = L.live[s];
                     V3 nrm, ptb;
@@ -939,7 +940,7 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                     }
                     if (m.n) manifold_refresh(m, A.ball.xf, c.xf);
                 }
-                mf_close(L, k);
+                mf_close(L, C, k, s);
             }
             if (d >= 1) {
                 // car d-1 vs car e-1: compound (car d) -> swapped compound
@@ -953,8 +954,8 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                 if (!aabb_overlap(dmn, dmx, emn, emx)) continue;
                 box_child_aabb(g, xe, emn, emx);
                 if (!aabb_overlap(emn, emx, dmn, dmx)) continue;
-                int k = mf_open(L, e, d, g.breaking);
-                int s = k >= 0 ? L.slot[k] : -1;
+                int s;
+                int k = mf_open(L, C, e, d, g.breaking, s);
                 if (s >= 0) {
                     Manifold& m = L.live[s];
                     const V3 side = (g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin}) * 2.f;
@@ -969,11 +970,12 @@ RSX_HD void arena_world_step(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>
                     }
                     manifold_refresh(m, ce.xf, cd.xf);
                 }
-                mf_close(L, k);
+                mf_close(L, C, k, s);
             }
         }
     }
 
+    mf_finish(L, C);
     if (!live) return;
     // ---- islands and manifold order (btSimulationIslandManager) ----
     int tag[MAXC + 1];
