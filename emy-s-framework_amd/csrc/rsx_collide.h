@@ -86,10 +86,21 @@ RSX_HD int sort_cached_points(const Manifold& m, const Contact& pt) {
 }
 
 // Add a point (normal on manifold body1 pointing to body0, point on body1,
-// signed distance); returns the index it landed at or -1 if rejected.
+// signed distance); returns the index it landed at or -1 if rejected.  The
+// _n form keeps the point count in `n` and the breaking threshold in `brk`
+// (registers of a caller adding several points; m.n is stored by it).
+RSX_HD int manifold_add_n(Manifold& m, int& n, float brk, const Xf& trA, const Xf& trB, V3 normal, V3 point_b,
+                          float depth, float friction, float restitution);
 RSX_HD int manifold_add(Manifold& m, const Xf& trA, const Xf& trB, V3 normal, V3 point_b, float depth, float friction,
                         float restitution) {
-    if (depth > m.breaking) return -1;
+    int n = m.n;
+    const int idx = manifold_add_n(m, n, m.breaking, trA, trB, normal, point_b, depth, friction, restitution);
+    m.n = n;
+    return idx;
+}
+RSX_HD int manifold_add_n(Manifold& m, int& n, float brk, const Xf& trA, const Xf& trB, V3 normal, V3 point_b,
+                          float depth, float friction, float restitution) {
+    if (depth > brk) return -1;
     V3 pa = point_b + normal * depth;
     Contact c;
     c.localA = inv_xform(trA, pa);
@@ -101,12 +112,12 @@ RSX_HD int manifold_add(Manifold& m, const Xf& trA, const Xf& trB, V3 normal, V3
     c.friction = friction;
     c.restitution = restitution;
     c.special = false;
-    int idx = m.n;
+    int idx = n;
     if (idx == kMaxManifoldPoints) {
         idx = sort_cached_points(m, c);
         if (idx < 0) idx = 0;
     } else {
-        m.n++;
+        n++;
     }
     m.pt[idx] = c;
     return idx;

@@ -226,6 +226,8 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, const TickScratch<MAX
     Xf trB = xf_identity();
     const PooledContact* pooled =
         T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
+    int mn = m.n;  // the manifold's point count and threshold in registers, m.n stored before the refresh
+    const float brk = m.breaking;
     int i = pos;
     int pk_next = i < ncand ? cand[i] : 0;
     for (; i < ncand; i++) {
@@ -238,8 +240,8 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, const TickScratch<MAX
             const PooledContact& r = pooled[W.pool_base + i];
             const int hit = r.hit;
             if (hit) {
-                int idx = manifold_add(m, b.xf, trB, r.normal_on_b, r.point_on_b, r.depth, minf(kBallFriction, 0.6f),
-                                       maxf(kBallRestitution, 0.3f));
+                int idx = manifold_add_n(m, mn, brk, b.xf, trB, r.normal_on_b, r.point_on_b, r.depth,
+                                         minf(kBallFriction, 0.6f), maxf(kBallRestitution, 0.3f));
                 if (idx >= 0) {
                     contact_added(A, T, m, idx);  // the edge correction ran with the query
                     if (hit & 2) contact_renormal(m.pt[idx], r.normal_fix);
@@ -249,15 +251,17 @@ RSX_HD void ball_vs_mesh(Arena<MAXC>& A, const TickCtx& T, const TickScratch<MAX
         }
         V3 v0, v1, v2;
         load_tri(T.mv, t, v0, v1, v2);
-        if (sphere_tri_early_out(b.xf.o, T.bg.radius, v0, v1, v2, m.breaking)) continue;
+        if (sphere_tri_early_out(b.xf.o, T.bg.radius, v0, v1, v2, brk)) continue;
         V3 pt, nrm;
         float depth;
-        if (sphere_triangle(b.xf.o, T.bg.radius, m.breaking, v0, v1, v2, pt, nrm, depth)) {
-            int idx = manifold_add(m, b.xf, trB, nrm, pt, depth, minf(kBallFriction, 0.6f), maxf(kBallRestitution, 0.3f));
+        if (sphere_triangle(b.xf.o, T.bg.radius, brk, v0, v1, v2, pt, nrm, depth)) {
+            int idx = manifold_add_n(m, mn, brk, b.xf, trB, nrm, pt, depth, minf(kBallFriction, 0.6f),
+                                     maxf(kBallRestitution, 0.3f));
             if (idx >= 0) contact_added(A, T, m, idx, t);
         }
     }
     pos = i;
+    m.n = mn;
     manifold_refresh(m, b.xf, trB);
 }
 
@@ -642,7 +646,9 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
     const Xf child = car_child_xf(g, c.xf);
     const Xf trB = xf_identity();
     const V3 hm = g.box_implicit + V3{g.box_margin, g.box_margin, g.box_margin};
-    const float max_d = g.box_margin + 0.f + m.breaking;
+    const float brk = m.breaking;  // with the point count, in registers (m.n stored before the refresh)
+    int mn = m.n;
+    const float max_d = g.box_margin + 0.f + brk;
     const PooledContact* pooled =
         T.pool && !W.pool_local ? (const PooledContact*)(T.pool + kPoolResOff) : nullptr;
     M3 inv_basis{};  // for the triangle early-out, which only the lane path runs
@@ -662,7 +668,7 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
                 const PooledContact& r = pooled[it];
                 const int hit = r.hit;
                 if (hit) {
-                    int idx = manifold_add(m, c.xf, trB, r.normal_on_b, r.point_on_b, r.depth, 0.6f, 0.3f);
+                    int idx = manifold_add_n(m, mn, brk, c.xf, trB, r.normal_on_b, r.point_on_b, r.depth, 0.6f, 0.3f);
                     if (idx >= 0) {
                         contact_added(A, T, m, idx);  // the edge correction ran with the query
                         if (hit & 2) contact_renormal(m.pt[idx], r.normal_fix);
@@ -673,17 +679,18 @@ RSX_HD void car_vs_mesh(Arena<MAXC>& A, const TickCtx& T, TickScratch<MAXC>& W, 
         }
         gjk::Shapes sh;
         load_tri(T.mv, t, sh.tri[0], sh.tri[1], sh.tri[2]);
-        if (!pooled && box_tri_separated(child, inv_basis, hm, sh.tri, m.breaking)) continue;
+        if (!pooled && box_tri_separated(child, inv_basis, hm, sh.tri, brk)) continue;
         sh.h = g.box_implicit;
         sh.margin = g.box_margin;
         gjk::Contact1 ct;
         if (gjk::closest_points(G, *T.sx, sh, child, trB, max_d * max_d, ct)) {
-            int idx = manifold_add(m, c.xf, trB, ct.normal_on_b, ct.point_on_b, ct.depth, 0.6f, 0.3f);
+            int idx = manifold_add_n(m, mn, brk, c.xf, trB, ct.normal_on_b, ct.point_on_b, ct.depth, 0.6f, 0.3f);
             if (idx >= 0) contact_added(A, T, m, idx, t);
         }
     }
     pos = i;
-    if (m.n) manifold_refresh(m, c.xf, trB);
+    m.n = mn;
+    if (mn) manifold_refresh(m, c.xf, trB);
   }
 }
 
